@@ -1,0 +1,182 @@
+/*
+ * nadvibs.h -- C ABI of the MI355X Lanczos engine for the diabatic vibronic
+ * Hamiltonian in a harmonic-oscillator direct-product basis.
+ *
+ * Every entry point returns an int32 status (NV_OK == 0) and never aborts or
+ * throws across the ABI; the text of the last error is available from
+ * nv_last_error().  Counts and indices are int64 (the reference is built with
+ * default 8-byte integers, makefile:14), reals are double.  Host vectors use
+ * the reference's Global-Arrays layout q(dimen, nstates) in Fortran order,
+ * i.e. C order [nstates][n_1]...[n_M] with the last mode fastest
+ * (Source/top.f90:379-386, :400-402), so a Fortran caller passes its arrays
+ * unchanged.  Iteration numbers are 1-based as in the reference.
+ *
+ * The context owns all device memory (operator tables, the Krylov basis,
+ * scratch); the caller owns every host array it passes.  Calls on one context
+ * are not thread-safe; use one context per host thread.
+ */
+#ifndef NADVIBS_H
+#define NADVIBS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define NV_OK 0
+#define NV_ERR_ARG 1     /* invalid argument or inconsistent sizes */
+#define NV_ERR_HIP 2     /* a HIP runtime call failed */
+#define NV_ERR_STATE 3   /* call out of sequence (e.g. iteration before init) */
+#define NV_ERR_IO 4      /* file could not be read or written */
+#define NV_ERR_NOMEM 5   /* device or host allocation failed */
+
+/* ---- operator conventions (reference quirks) --------------------------- */
+/* <m|q^3|m+1> taken as sqrt((m+1)^3/8), one third of the exact value
+ * (Source/function.f90:225). */
+#define NV_QUIRK_Q3ELEM 1
+/* The vibrationally diagonal part of an inter-state block (constant term and
+ * the Delta=0 part of all-even monomials) is applied only for row state >
+ * column state (Source/top.f90:545-553, :752-754). */
+#define NV_QUIRK_LOWERDIAG 2
+/* Reference behaviour: both of the above. */
+#define NV_QUIRKS_REF 3
+/* Exact, symmetric operator. */
+#define NV_QUIRKS_EXACT 0
+
+/* ---- H*v precision ----------------------------------------------------- */
+#define NV_PREC_F64 0    /* off-diagonal couplings in FP64 */
+#define NV_PREC_F32 1    /* off-diagonal couplings in FP32, diagonal FP64 */
+#define NV_PREC_BF16 2   /* vector streamed as BF16 hi+lo pair, FP32 arithmetic, diagonal FP64 */
+
+/* ---- problem description ---------------------------------------------- */
+/* The data of basis.in / nadvibs.in after reading, before the reference's
+ * term numbering (Source/top.f90:88-246 performs that numbering; the
+ * library repeats it from these raw arrays). */
+typedef struct nv_problem {
+    int64_t nstates;          /* electronic states */
+    int64_t nmodes;           /* vibrational modes */
+    int64_t ordr;             /* order of the expansion (1..4 supported by hoelem) */
+    const int64_t *nfunc;     /* [nmodes] basis functions per mode ("basis") */
+    const double *aomega;     /* [nmodes] harmonic frequencies, hartree */
+    const double *concoef;    /* [nstblks] constant term of each state block,
+                                 block order (1,1),(2,1),(2,2),(3,1),... */
+    const double *potterms;   /* POTterms(noterms(ordr), ordr, nstblks), Fortran
+                                 order: for block b, order p the noterms(p)
+                                 coefficients in nadvibs.in order, the rest of
+                                 the noterms(ordr) slots ignored */
+    double ztoler;            /* |c| < ztoler dropped (basis.in ztoler) */
+    int32_t quirks;           /* NV_QUIRK_* bits */
+    int32_t neworigin;        /* basis.in shiftref != 0: Franck-Condon start */
+    const double *statew;     /* [nstates] state weights, NULL = all 1 */
+    const double *bomega;     /* [nmodes]  (neworigin only) */
+    const double *dvec;       /* [nmodes]  (neworigin only) */
+    const double *tmat;       /* [nmodes*nmodes] T, column-major (neworigin only) */
+    int64_t nirreps;          /* irreducible representations (>=1) */
+    const int64_t *npirrep;   /* [nirreps] modes per irrep, NULL = one irrep */
+    const int64_t *istate;    /* [nmodes] initial-state quanta, NULL = zeros */
+} nv_problem;
+
+typedef struct nv_ctx nv_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Builds the operator (replaces read_constants' term numbering
+ * Source/top.f90:150-234, hoelem table :501-527, diagonal block list
+ * :528-553 and compute_allIndex Source/secondary.f90:3-226) on device
+ * `device`.  *out is NULL on failure. */
+int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out);
+int32_t nv_destroy(nv_ctx *ctx);
+/* Text of the last error on ctx (or of the last failed nv_create when ctx is
+ * NULL).  Valid until the next call on the same context. */
+const char *nv_last_error(const nv_ctx *ctx);
+
+/* dimen = prod nfunc, n = nstates*dimen. */
+int32_t nv_dims(const nv_ctx *ctx, int64_t *dimen, int64_t *n);
+/* Number of stored terms per number of distinct modes (nztrms,
+ * Source/top.f90:150-201): counts[ordr]. */
+int32_t nv_term_summary(const nv_ctx *ctx, int64_t *nztrms);
+/* The reference's term-count self check (Source/top.f90:438-499): number of
+ * non-zero matrix elements ntcom(blk, i) per unique state block and number of
+ * distinct modes; out[(i-1)*nstblks + (blk-1)]. */
+int32_t nv_term_counts(const nv_ctx *ctx, int64_t *out);
+
+/* ---- matrix-free H*v (replaces Hv, Source/top.f90:672-818) -------------- */
+/* y = H x for host arrays x, y of n doubles. */
+int32_t nv_hv(nv_ctx *ctx, int32_t prec, const double *x, double *y);
+/* y = H x for device arrays of n doubles, enqueued on `stream` (a
+ * hipStream_t, NULL = the context's stream).  No host synchronisation. */
+int32_t nv_hv_device(nv_ctx *ctx, int32_t prec, const double *x_dev,
+                     double *y_dev, void *stream);
+
+/* ---- start vector (replaces generate_initialvec,
+ *      Source/secondary.f90:351-809) ------------------------------------ */
+/* Builds the normalised start vector on device; copies it to q (n doubles)
+ * when q is not NULL. */
+int32_t nv_start_vector(nv_ctx *ctx, double *q);
+
+/* ---- Lanczos (replaces the loop of Source/main.f90:63-71) ---------------- */
+typedef struct nv_lanczos_opts {
+    int64_t niter;            /* total iterations (basis.in niter) */
+    int32_t reorthog;         /* basis.in reorthog: 0 none, 1 partial with the
+                                 omega recurrence, >1 partial with exact overlaps */
+    int32_t prec;             /* NV_PREC_* for H*v */
+    int64_t chkorthog;        /* exact overlap check period (default 100) */
+    uint64_t seed;            /* seed of the omega-estimate noise */
+    double noise_scale;       /* multiplies every noise draw (1 = reference, 0 = off) */
+} nv_lanczos_opts;
+
+/* Allocates the Krylov basis for opts->niter iterations, builds the start
+ * vector and beta(0) (Source/top.f90:590-613). */
+int32_t nv_lanczos_init(nv_ctx *ctx, const nv_lanczos_opts *opts);
+/* Per-iteration entry points, in the reference's order (Source/main.f90:64-70):
+ * update_lanczos (:646-670), Hv (:672-818), lanczos_step (:820-842),
+ * set_omega (:844-908), partial_orthog (:911-1005).  All are asynchronous. */
+int32_t nv_update_lanczos(nv_ctx *ctx, int64_t iter);
+int32_t nv_hv_iter(nv_ctx *ctx, int64_t iter);
+int32_t nv_lanczos_step(nv_ctx *ctx, int64_t iter);
+int32_t nv_set_omega(nv_ctx *ctx, int64_t iter);
+int32_t nv_partial_orthog(nv_ctx *ctx, int64_t iter);
+/* Runs iterations iiter..niter of the loop above with the per-iteration work
+ * captured in a HIP graph.  Asynchronous. */
+int32_t nv_lanczos_run(nv_ctx *ctx, int64_t iiter, int64_t niter);
+/* Copies alpha(1:niter), beta(0:niter) and, when omega is not NULL,
+ * omega(2, 0:niter) (Fortran order) to the host.  Synchronises. */
+int32_t nv_get_coefficients(nv_ctx *ctx, double *alpha, double *beta,
+                            double *omega);
+/* Number of partial re-orthogonalisations performed and Krylov vectors
+ * touched by them so far. */
+int32_t nv_pro_stats(const nv_ctx *ctx, int64_t *northog_events,
+                     int64_t *vectors_touched);
+/* Copies Lanczos vector q_j (1 <= j <= last iteration + 1; j = last+1 gives
+ * the unnormalised residual) to the host. */
+int32_t nv_get_lanczos_vector(nv_ctx *ctx, int64_t j, double *q);
+
+/* ---- tridiagonal eigensolve (replaces compute_ritzsystem + GIVENS,
+ *      Source/secondary.f90:240-257, Source/yarkony.f90:91-543) ---------- */
+/* Eigenvalues (ascending) of the n x n tridiagonal T with diagonal
+ * alpha[0..n-1] and off-diagonal beta[1..n-1] (beta[0] = beta(0) unused),
+ * intensities inten[i] = v(1,i)^2, convergence bji[i] = |beta[n] * v(n,i)|
+ * and sign[i] = sign of v(1,i).  Runs on the context's device. */
+int32_t nv_ritz(nv_ctx *ctx, int64_t n, const double *alpha,
+                const double *beta, double *eval, double *inten, double *bji,
+                int32_t *sign);
+
+/* ---- restart files (make_restart / load_restartinfo,
+ *      Source/top.f90:1007-1049, Source/secondary.f90:872-892) -------------- */
+/* Writes q1, q2, q3 of the last iteration as nstates records each of dimen
+ * doubles (the layout of the reference's nadvibs.35.<rank> direct-access
+ * file for one process). */
+int32_t nv_save_vectors(nv_ctx *ctx, const char *path);
+/* Restores the state saved by nv_save_vectors together with alpha(1:iter),
+ * beta(0:iter) (and omega when non-NULL), so that nv_update_lanczos(iter+1)
+ * continues the run. */
+int32_t nv_load_vectors(nv_ctx *ctx, const char *path, int64_t iter,
+                        const double *alpha, const double *beta,
+                        const double *omega);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NADVIBS_H */

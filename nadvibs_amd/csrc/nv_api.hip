@@ -1,0 +1,525 @@
+// C ABI of the engine (include/nadvibs.h).  Owns the context: operator
+// tables on the device, the Krylov basis and every scratch buffer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nadvibs.h"
+#include "nv_kernels.hpp"
+#include "nv_operator.hpp"
+#include "nv_pro.hpp"
+
+struct nv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    nv::Operator op;
+    NvOpDev dev{};
+    std::vector<void *> allocs;  // operator tables
+
+    // standalone H*v buffers
+    double *hx = nullptr, *hy = nullptr;
+
+    // Lanczos run
+    bool lz_ready = false;
+    nv_lanczos_opts opts{};
+    nv::LzState st{};
+    int64_t *d_iter = nullptr;
+    double *d_startvals = nullptr;
+    int64_t last_iter = 0;       // last iteration whose lanczos_step was issued
+    int64_t next_update = 1;     // iteration expected by update_lanczos
+    std::vector<void *> lz_allocs;
+    nv::ProState pro{};
+
+    // graph of one iteration
+    hipGraphExec_t graph = nullptr;
+};
+
+static std::string g_create_error;
+
+namespace {
+
+int32_t fail(nv_ctx *c, int32_t code, const std::string &msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int32_t hip_fail(nv_ctx *c, hipError_t e, const char *where) {
+    return fail(c, NV_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define NV_HIP(c, call)                                      \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #call); \
+    } while (0)
+
+template <typename T>
+hipError_t upload(nv_ctx *c, const std::vector<T> &h, const T **out, std::vector<void *> &list) {
+    *out = nullptr;
+    if (h.empty()) return hipSuccess;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, h.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    list.push_back(p);
+    e = hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    *out = reinterpret_cast<const T *>(p);
+    return e;
+}
+
+void free_list(std::vector<void *> &l) {
+    for (void *p : l) hipFree(p);
+    l.clear();
+}
+
+void release_lanczos(nv_ctx *c) {
+    if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    free_list(c->lz_allocs);
+    c->st = nv::LzState{};
+    c->pro = nv::ProState{};
+    c->lz_ready = false;
+}
+
+bool valid_prec(int32_t p) { return p == NV_PREC_F64 || p == NV_PREC_F32 || p == NV_PREC_BF16; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
+    if (!out) return fail(nullptr, NV_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!prob) return fail(nullptr, NV_ERR_ARG, "problem is NULL");
+    nv_ctx *c = new (std::nothrow) nv_ctx();
+    if (!c) return fail(nullptr, NV_ERR_NOMEM, "host allocation failed");
+    std::string err;
+    if (!nv::build_operator(*prob, c->op, err)) {
+        delete c;
+        return fail(nullptr, NV_ERR_ARG, err);
+    }
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    const nv::Operator &op = c->op;
+    NvOpDev &d = c->dev;
+    d.ns = (int32_t)op.ns; d.M = (int32_t)op.M; d.D = (int32_t)op.D; d.N = (int32_t)op.N;
+    d.nmax = (int32_t)op.nmax; d.ncodes = op.ncodes;
+    d.nterms = (int32_t)op.dev_terms.size(); d.ndterms = (int32_t)op.dev_dterms.size();
+    std::vector<float> cm32(op.cmat.begin(), op.cmat.end()), ft32(op.ftab.begin(), op.ftab.end());
+    std::vector<int32_t> nf32(op.nfunc.begin(), op.nfunc.end());
+    if (e == hipSuccess) e = upload(c, op.dev_terms, &d.terms, c->allocs);
+    if (e == hipSuccess) e = upload(c, op.dev_steps, &d.steps, c->allocs);
+    if (e == hipSuccess) e = upload(c, op.cmat, &d.cmat64, c->allocs);
+    if (e == hipSuccess) e = upload(c, cm32, &d.cmat32, c->allocs);
+    if (e == hipSuccess) e = upload(c, op.ftab, &d.ftab64, c->allocs);
+    if (e == hipSuccess) e = upload(c, ft32, &d.ftab32, c->allocs);
+    if (e == hipSuccess) e = upload(c, op.dev_dterms, &d.dterms, c->allocs);
+    if (e == hipSuccess) e = upload(c, op.dcmat, &d.dcmat, c->allocs);
+    if (e == hipSuccess) e = upload(c, op.dconst, &d.dconst, c->allocs);
+    if (e == hipSuccess) e = upload(c, op.omega, &d.omega, c->allocs);
+    if (e == hipSuccess) e = upload(c, nf32, &d.nfunc, c->allocs);
+    if (e != hipSuccess) {
+        g_create_error = std::string("device setup failed: ") + hipGetErrorString(e);
+        free_list(c->allocs);
+        if (c->stream) hipStreamDestroy(c->stream);
+        delete c;
+        return NV_ERR_HIP;
+    }
+    if (nv::hv_lds_bytes(d, NV_PREC_F64) > 64 * 1024) {
+        free_list(c->allocs);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return fail(nullptr, NV_ERR_ARG, "factor table exceeds the LDS budget of the H*v kernel");
+    }
+    *out = c;
+    return NV_OK;
+}
+
+int32_t nv_destroy(nv_ctx *c) {
+    if (!c) return NV_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    release_lanczos(c);
+    free_list(c->allocs);
+    if (c->hx) hipFree(c->hx);
+    if (c->hy) hipFree(c->hy);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return NV_OK;
+}
+
+const char *nv_last_error(const nv_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int32_t nv_dims(const nv_ctx *c, int64_t *dimen, int64_t *n) {
+    if (!c) return NV_ERR_ARG;
+    if (dimen) *dimen = c->op.D;
+    if (n) *n = c->op.N;
+    return NV_OK;
+}
+
+int32_t nv_term_summary(const nv_ctx *c, int64_t *nztrms) {
+    if (!c || !nztrms) return NV_ERR_ARG;
+    for (int64_t i = 0; i < c->op.ordr; ++i) nztrms[i] = c->op.nztrms[i];
+    return NV_OK;
+}
+
+int32_t nv_term_counts(const nv_ctx *c, int64_t *out) {
+    if (!c || !out) return NV_ERR_ARG;
+    for (size_t i = 0; i < c->op.ntcom.size(); ++i) out[i] = c->op.ntcom[i];
+    return NV_OK;
+}
+
+int32_t nv_hv_device(nv_ctx *c, int32_t prec, const double *x, double *y, void *stream) {
+    if (!c) return NV_ERR_ARG;
+    if (!x || !y) return fail(c, NV_ERR_ARG, "x or y is NULL");
+    if (prec != NV_PREC_F64 && prec != NV_PREC_F32)
+        return fail(c, NV_ERR_ARG, "nv_hv supports NV_PREC_F64 and NV_PREC_F32");
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, nv::launch_hv_full(c->dev, prec, x, y, s));
+    return NV_OK;
+}
+
+int32_t nv_hv(nv_ctx *c, int32_t prec, const double *x, double *y) {
+    if (!c) return NV_ERR_ARG;
+    if (!x || !y) return fail(c, NV_ERR_ARG, "x or y is NULL");
+    NV_HIP(c, hipSetDevice(c->device));
+    const size_t bytes = (size_t)c->op.N * sizeof(double);
+    if (!c->hx) {
+        NV_HIP(c, hipMalloc(&c->hx, bytes));
+        NV_HIP(c, hipMalloc(&c->hy, bytes));
+    }
+    NV_HIP(c, hipMemcpyAsync(c->hx, x, bytes, hipMemcpyHostToDevice, c->stream));
+    int32_t rc = nv_hv_device(c, prec, c->hx, c->hy, c->stream);
+    if (rc != NV_OK) return rc;
+    NV_HIP(c, hipMemcpyAsync(y, c->hy, bytes, hipMemcpyDeviceToHost, c->stream));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    return NV_OK;
+}
+
+// ---- start vector -------------------------------------------------------
+// Default start (shiftref = 0, Source/secondary.f90:727-783): |0> in every
+// state, weighted by statew / sum(statew), normalised.  The Franck-Condon
+// start (shiftref != 0) is built by nv::build_fc_start.
+static int32_t build_start(nv_ctx *c, double *dst) {
+    const nv::Operator &op = c->op;
+    if (op.neworigin) {
+        hipError_t e = nv::build_fc_start(op, c->dev, dst, c->stream, c->err);
+        if (e != hipSuccess) return c->err.empty() ? hip_fail(c, e, "fc start") : NV_ERR_HIP;
+        return NV_OK;
+    }
+    std::vector<double> vals(op.ns);
+    // q3 = 1 at index 1 of every state; d1 = sqrt(q3.q3/nstates) = 1
+    double d2 = 0.0;
+    for (int64_t s = 0; s < op.ns; ++s) d2 += op.statew[s];
+    if (d2 == 0.0) d2 = 1.0;
+    double tot = 0.0;
+    for (int64_t s = 0; s < op.ns; ++s) {
+        vals[s] = 1.0 * (op.statew[s] / (d2 * 1.0));
+        tot += vals[s] * vals[s];
+    }
+    const double sc = 1.0 / std::sqrt(tot);
+    for (int64_t s = 0; s < op.ns; ++s) vals[s] *= sc;
+    if (!c->d_startvals) NV_HIP(c, hipMalloc(&c->d_startvals, NV_MAXS * sizeof(double)));
+    NV_HIP(c, hipMemcpyAsync(c->d_startvals, vals.data(), op.ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    NV_HIP(c, nv::launch_fill_start(dst, op.D, (int)op.ns, c->d_startvals, c->stream));
+    return NV_OK;
+}
+
+int32_t nv_start_vector(nv_ctx *c, double *q) {
+    if (!c) return NV_ERR_ARG;
+    NV_HIP(c, hipSetDevice(c->device));
+    const size_t bytes = (size_t)c->op.N * sizeof(double);
+    double *tmp = nullptr;
+    NV_HIP(c, hipMalloc(&tmp, bytes));
+    int32_t rc = build_start(c, tmp);
+    if (rc == NV_OK && q) {
+        hipError_t e = hipMemcpyAsync(q, tmp, bytes, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "start vector copy");
+    }
+    hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    return rc;
+}
+
+// ---- Lanczos --------------------------------------------------------------
+int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
+    if (!c || !o) return NV_ERR_ARG;
+    if (o->niter < 1) return fail(c, NV_ERR_ARG, "niter must be >= 1");
+    if (!valid_prec(o->prec)) return fail(c, NV_ERR_ARG, "unknown precision");
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    release_lanczos(c);
+    c->opts = *o;
+    if (c->opts.chkorthog < 1) c->opts.chkorthog = 100;
+    const int64_t N = c->op.N;
+    const bool keep = o->reorthog > 0;
+    nv::LzState &st = c->st;
+    st.nslots = keep ? o->niter + 2 : 3;
+    auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e == hipSuccess) c->lz_allocs.push_back(*p);
+        return e;
+    };
+    NV_HIP(c, alloc((size_t)st.nslots * N * sizeof(double), (void **)&st.V));
+    const size_t el = o->prec == NV_PREC_F64 ? 8 : (o->prec == NV_PREC_F32 ? 4 : 2);
+    if (o->prec != NV_PREC_F64) NV_HIP(c, alloc((size_t)N * el, &st.vlo));
+    NV_HIP(c, alloc((size_t)N * (o->prec == NV_PREC_F64 ? 8 : 4), &st.y));
+    NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.alpha));
+    NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.beta));
+    const int64_t nparts = std::max<int64_t>(nv::hv_blocks(c->dev), 8192) + 64;
+    NV_HIP(c, alloc(nparts * sizeof(double), (void **)&st.partials));
+    NV_HIP(c, alloc(nv::kNumCounters * sizeof(unsigned), (void **)&st.counters));
+    NV_HIP(c, alloc(sizeof(int64_t), (void **)&c->d_iter));
+    st.iter = c->d_iter;
+    NV_HIP(c, hipMemsetAsync(st.counters, 0, nv::kNumCounters * sizeof(unsigned), c->stream));
+    NV_HIP(c, hipMemsetAsync(st.alpha, 0, (o->niter + 3) * sizeof(double), c->stream));
+    NV_HIP(c, hipMemsetAsync(st.beta, 0, (o->niter + 3) * sizeof(double), c->stream));
+    NV_HIP(c, hipMemsetAsync(st.V, 0, (size_t)std::min<int64_t>(st.nslots, 3) * N * sizeof(double), c->stream));
+    if (keep) {
+        hipError_t e = nv::pro_init(c->pro, c->opts, c->op, c->dev, st, c->lz_allocs, c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "pro_init");
+    }
+    // start vector in the slot of iteration 1, beta(0) = |start|
+    double *s1 = st.V + (1 % st.nslots) * N;
+    int32_t rc = build_start(c, s1);
+    if (rc != NV_OK) return rc;
+    NV_HIP(c, nv::launch_norm_to(s1, N, st.beta, st.partials, st.counters + nv::kCntMisc, c->stream));
+    c->lz_ready = true;
+    c->last_iter = 0;
+    c->next_update = 1;
+    return NV_OK;
+}
+
+static int32_t check_iter(nv_ctx *c, int64_t it) {
+    if (!c) return NV_ERR_ARG;
+    if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
+    if (it < 1 || it > c->opts.niter) return fail(c, NV_ERR_ARG, "iteration out of range");
+    return NV_OK;
+}
+
+int32_t nv_update_lanczos(nv_ctx *c, int64_t it) {
+    int32_t rc = check_iter(c, it);
+    if (rc) return rc;
+    if (it != c->next_update) return fail(c, NV_ERR_STATE, "update_lanczos called out of order");
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, nv::launch_set_iter(c->d_iter, it, 0, c->stream));
+    NV_HIP(c, nv::launch_normalize(c->dev, c->opts.prec, c->st, c->stream));
+    c->next_update = it + 1;
+    return NV_OK;
+}
+
+int32_t nv_hv_iter(nv_ctx *c, int64_t it) {
+    int32_t rc = check_iter(c, it);
+    if (rc) return rc;
+    if (it != c->next_update - 1) return fail(c, NV_ERR_STATE, "Hv called before update_lanczos");
+    NV_HIP(c, nv::launch_hv_iter(c->dev, c->opts.prec, c->st, c->stream));
+    return NV_OK;
+}
+
+int32_t nv_lanczos_step(nv_ctx *c, int64_t it) {
+    int32_t rc = check_iter(c, it);
+    if (rc) return rc;
+    if (it != c->next_update - 1) return fail(c, NV_ERR_STATE, "lanczos_step called before update_lanczos");
+    NV_HIP(c, nv::launch_lanczos_step(c->dev, c->opts.prec, c->st, c->stream));
+    c->last_iter = it;
+    return NV_OK;
+}
+
+int32_t nv_set_omega(nv_ctx *c, int64_t it) {
+    int32_t rc = check_iter(c, it);
+    if (rc) return rc;
+    if (c->opts.reorthog <= 0 || it <= 1) return NV_OK;
+    NV_HIP(c, nv::pro_set_omega(c->pro, c->dev, c->st, it, c->stream));
+    return NV_OK;
+}
+
+int32_t nv_partial_orthog(nv_ctx *c, int64_t it) {
+    int32_t rc = check_iter(c, it);
+    if (rc) return rc;
+    if (c->opts.reorthog <= 0 || it <= 1) return NV_OK;
+    NV_HIP(c, nv::pro_partial_orthog(c->pro, c->dev, c->st, c->stream));
+    return NV_OK;
+}
+
+int32_t nv_lanczos_run(nv_ctx *c, int64_t iiter, int64_t niter) {
+    if (!c) return NV_ERR_ARG;
+    if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
+    if (iiter != c->next_update) return fail(c, NV_ERR_STATE, "run must start at the next iteration");
+    if (niter > c->opts.niter) return fail(c, NV_ERR_ARG, "niter beyond the initialised run");
+    for (int64_t i = iiter; i <= niter; ++i) {
+        int32_t rc = nv_update_lanczos(c, i);
+        if (!rc) rc = nv_hv_iter(c, i);
+        if (!rc) rc = nv_lanczos_step(c, i);
+        if (!rc && c->opts.reorthog > 0 && i > 1) {
+            rc = nv_set_omega(c, i);
+            if (!rc) rc = nv_partial_orthog(c, i);
+        }
+        if (rc) return rc;
+    }
+    return NV_OK;
+}
+
+int32_t nv_get_coefficients(nv_ctx *c, double *alpha, double *beta, double *omega) {
+    if (!c) return NV_ERR_ARG;
+    if (!c->lz_ready) return fail(c, NV_ERR_STATE, "no Lanczos run");
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t n = c->last_iter;
+    if (alpha && n > 0) NV_HIP(c, hipMemcpy(alpha, c->st.alpha + 1, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (beta) NV_HIP(c, hipMemcpy(beta, c->st.beta, (n + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    if (omega) {
+        if (c->opts.reorthog <= 0) return fail(c, NV_ERR_STATE, "omega is only kept with reorthog > 0");
+        hipError_t e = nv::pro_get_omega(c->pro, omega, n);
+        if (e != hipSuccess) return hip_fail(c, e, "omega copy");
+    }
+    return NV_OK;
+}
+
+int32_t nv_pro_stats(const nv_ctx *c, int64_t *events, int64_t *touched) {
+    if (!c) return NV_ERR_ARG;
+    int64_t ev = 0, tv = 0;
+    if (c->lz_ready && c->opts.reorthog > 0) {
+        hipStreamSynchronize(c->stream);
+        nv::pro_stats(c->pro, &ev, &tv);
+    }
+    if (events) *events = ev;
+    if (touched) *touched = tv;
+    return NV_OK;
+}
+
+int32_t nv_get_lanczos_vector(nv_ctx *c, int64_t j, double *q) {
+    if (!c || !q) return NV_ERR_ARG;
+    if (!c->lz_ready) return fail(c, NV_ERR_STATE, "no Lanczos run");
+    const int64_t last = c->last_iter;
+    if (j < 1 || j > last + 1) return fail(c, NV_ERR_ARG, "vector index out of range");
+    if (c->st.nslots == 3 && j < last - 1) return fail(c, NV_ERR_STATE, "Krylov basis not kept (reorthog = 0)");
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    const double *src = c->st.V + (j % c->st.nslots) * c->op.N;
+    NV_HIP(c, hipMemcpy(q, src, c->op.N * sizeof(double), hipMemcpyDeviceToHost));
+    return NV_OK;
+}
+
+// ---- tridiagonal eigensolve ---------------------------------------------
+int32_t nv_ritz(nv_ctx *c, int64_t n, const double *alpha, const double *beta, double *eval, double *inten,
+                double *bji, int32_t *sign) {
+    if (!c) return NV_ERR_ARG;
+    if (n < 1 || !alpha || !beta || !eval) return fail(c, NV_ERR_ARG, "bad tridiagonal arguments");
+    NV_HIP(c, hipSetDevice(c->device));
+    std::vector<double> root(n), v1(n), vn(n);
+    // Frobenius scaling in the order GIVENS accumulates it (Source/yarkony.f90:221-247)
+    double factor = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        factor = std::fmax(factor, std::fabs(alpha[i]));
+        if (i > 0) factor = std::fmax(factor, std::fabs(beta[i]));
+    }
+    double anorm = 0.0;
+    if (n == 1) {
+        root[0] = alpha[0]; v1[0] = 1.0; vn[0] = 1.0;
+    } else if (factor == 0.0) {
+        for (int64_t i = 0; i < n; ++i) { root[i] = 0.0; v1[i] = i == 0; vn[i] = i == n - 1; }
+    } else {
+        for (int64_t i = 0; i < n; ++i) {
+            if (i > 0) anorm += (beta[i] / factor) * (beta[i] / factor);
+            anorm += (alpha[i] / factor) * (alpha[i] / factor) / 2.0;
+        }
+        anorm = std::sqrt(anorm * 2.0) * factor;
+        std::vector<double> d(n), e(n, 0.0), e2(n, 0.0);
+        for (int64_t i = 0; i < n; ++i) d[i] = alpha[i] / anorm;
+        for (int64_t i = 0; i + 1 < n; ++i) { e[i] = beta[i + 1] / anorm; e2[i] = e[i] * e[i]; }
+        nv::TriWork w;
+        std::vector<void *> tmp;
+        auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+            hipError_t er = hipMalloc(p, bytes);
+            if (er == hipSuccess) tmp.push_back(*p);
+            return er;
+        };
+        hipError_t er = alloc(n * 8, (void **)&w.d);
+        if (er == hipSuccess) er = alloc(n * 8, (void **)&w.e);
+        if (er == hipSuccess) er = alloc(n * 8, (void **)&w.e2);
+        if (er == hipSuccess) er = alloc(n * 8, (void **)&w.root);
+        if (er == hipSuccess) er = alloc((size_t)n * n * 8, (void **)&w.vec);
+        if (er == hipSuccess) er = alloc((size_t)4 * n * n * 8, (void **)&w.lu);
+        if (er == hipSuccess) er = alloc(n * 4, (void **)&w.ia);
+        if (er == hipSuccess) er = alloc(n * 8, (void **)&w.rank);
+        if (er == hipSuccess) er = hipMemcpyAsync(w.d, d.data(), n * 8, hipMemcpyHostToDevice, c->stream);
+        if (er == hipSuccess) er = hipMemcpyAsync(w.e, e.data(), n * 8, hipMemcpyHostToDevice, c->stream);
+        if (er == hipSuccess) er = hipMemcpyAsync(w.e2, e2.data(), n * 8, hipMemcpyHostToDevice, c->stream);
+        if (er == hipSuccess) er = nv::launch_tridiag(n, w, anorm, c->stream);
+        if (er == hipSuccess) er = hipMemcpyAsync(root.data(), w.root, n * 8, hipMemcpyDeviceToHost, c->stream);
+        if (er == hipSuccess) er = hipMemcpyAsync(v1.data(), w.vec, n * 8, hipMemcpyDeviceToHost, c->stream);
+        if (er == hipSuccess)
+            er = hipMemcpyAsync(vn.data(), w.vec + (n - 1) * n, n * 8, hipMemcpyDeviceToHost, c->stream);
+        if (er == hipSuccess) er = hipStreamSynchronize(c->stream);
+        for (void *p : tmp) hipFree(p);
+        if (er != hipSuccess) return hip_fail(c, er, "tridiagonal eigensolve");
+        for (int64_t i = 0; i < n; ++i) root[i] *= anorm;
+    }
+    const double bn = beta[n];
+    for (int64_t i = 0; i < n; ++i) {
+        eval[i] = root[i];
+        if (inten) inten[i] = v1[i] * v1[i];
+        if (bji) bji[i] = std::fabs(bn * vn[i]);
+        if (sign) sign[i] = v1[i] < 0 ? -1 : 1;
+    }
+    return NV_OK;
+}
+
+// ---- restart vectors ------------------------------------------------------
+int32_t nv_save_vectors(nv_ctx *c, const char *path) {
+    if (!c || !path) return NV_ERR_ARG;
+    if (!c->lz_ready || c->last_iter < 1) return fail(c, NV_ERR_STATE, "no completed iteration");
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t N = c->op.N, it = c->last_iter;
+    std::vector<double> buf(3 * N, 0.0);
+    const int64_t js[3] = {it - 1, it, it + 1};
+    for (int k = 0; k < 3; ++k) {
+        if (js[k] < 1) continue;
+        const double *src = c->st.V + (js[k] % c->st.nslots) * N;
+        NV_HIP(c, hipMemcpy(buf.data() + k * N, src, N * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(c, NV_ERR_IO, std::string("cannot open ") + path);
+    size_t w = std::fwrite(buf.data(), sizeof(double), buf.size(), f);
+    std::fclose(f);
+    if (w != buf.size()) return fail(c, NV_ERR_IO, std::string("short write to ") + path);
+    return NV_OK;
+}
+
+int32_t nv_load_vectors(nv_ctx *c, const char *path, int64_t iter, const double *alpha, const double *beta,
+                        const double *omega) {
+    if (!c || !path || !alpha || !beta) return NV_ERR_ARG;
+    if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
+    if (iter < 1 || iter >= c->opts.niter) return fail(c, NV_ERR_ARG, "restart iteration out of range");
+    if (c->opts.reorthog > 0) return fail(c, NV_ERR_STATE, "restart with re-orthogonalisation needs the archive");
+    (void)omega;
+    const int64_t N = c->op.N;
+    std::vector<double> buf(3 * N);
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(c, NV_ERR_IO, std::string("cannot open ") + path);
+    size_t r = std::fread(buf.data(), sizeof(double), buf.size(), f);
+    std::fclose(f);
+    if (r != buf.size()) return fail(c, NV_ERR_IO, std::string("short read from ") + path);
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t js[3] = {iter - 1, iter, iter + 1};
+    for (int k = 0; k < 3; ++k) {
+        double *dst = c->st.V + (js[k] % c->st.nslots) * N;
+        NV_HIP(c, hipMemcpy(dst, buf.data() + k * N, N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    NV_HIP(c, hipMemcpy(c->st.alpha + 1, alpha, iter * sizeof(double), hipMemcpyHostToDevice));
+    NV_HIP(c, hipMemcpy(c->st.beta, beta, (iter + 1) * sizeof(double), hipMemcpyHostToDevice));
+    c->last_iter = iter;
+    c->next_update = iter + 1;
+    return NV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,639 @@
+// HIP kernels of the Lanczos path for gfx950 (64-wide wavefronts).
+//
+//  * hv_*          matrix-free H*v over the state-major tensor x[s][n_1..n_M]
+//                  (last mode fastest): one thread per vibrational index r
+//                  computes all states; the mode digits of r and the factor
+//                  table sit in LDS; coupling matrices are read as uniform
+//                  (scalar) loads.
+//  * normalize / lanczos_step   the three-term recurrence in FP64 with the
+//                  dots reduced per block and finished by the last block to
+//                  arrive (fixed summation order: reproducible run to run).
+//  * tri_*         Sturm bisection and inverse iteration for the tridiagonal
+//                  eigenproblem, one root per thread.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include "nv_kernels.hpp"
+
+namespace nv {
+
+// ------------------------------------------------------------------------
+// reductions
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-wide sum; every thread gets the result.  blockDim.x <= 1024.
+__device__ double block_sum(double v) {
+    __shared__ double red[16];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += red[i];
+    return t;
+}
+
+// Each block contributes `v`; the last block to finish reduces the per-block
+// partials in block order and returns true on every thread of that block
+// with *total set.  The counter is re-armed by the last block.
+__device__ bool last_block_total(double v, double *partials, unsigned *counter, double *total) {
+    __shared__ bool is_last;
+    __shared__ double tot;
+    double b = block_sum(v);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[blockIdx.x], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        is_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double s = 0.0;
+    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+        s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        tot = s;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    *total = tot;
+    return true;
+}
+
+// ------------------------------------------------------------------------
+// element types
+// ------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ float to_f(T v);
+template <> __device__ __forceinline__ float to_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ float to_f<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+
+template <typename TIn, typename TAcc> __device__ __forceinline__ TAcc load_acc(const TIn *p);
+template <> __device__ __forceinline__ double load_acc<double, double>(const double *p) { return *p; }
+template <> __device__ __forceinline__ float load_acc<float, float>(const float *p) { return *p; }
+template <> __device__ __forceinline__ float load_acc<__hip_bfloat16, float>(const __hip_bfloat16 *p) {
+    return __bfloat162float(*p);
+}
+template <> __device__ __forceinline__ float load_acc<double, float>(const double *p) { return (float)*p; }
+
+template <typename T> __device__ __forceinline__ T from_d(double v);
+template <> __device__ __forceinline__ double from_d<double>(double v) { return v; }
+template <> __device__ __forceinline__ float from_d<float>(double v) { return (float)v; }
+template <> __device__ __forceinline__ __hip_bfloat16 from_d<__hip_bfloat16>(double v) {
+    return __float2bfloat16((float)v);
+}
+
+template <typename T> __device__ __forceinline__ const T *ftab_of(const NvOpDev &op);
+template <> __device__ __forceinline__ const double *ftab_of<double>(const NvOpDev &op) { return op.ftab64; }
+template <> __device__ __forceinline__ const float *ftab_of<float>(const NvOpDev &op) { return op.ftab32; }
+template <typename T> __device__ __forceinline__ const T *cmat_of(const NvOpDev &op);
+template <> __device__ __forceinline__ const double *cmat_of<double>(const NvOpDev &op) { return op.cmat64; }
+template <> __device__ __forceinline__ const float *cmat_of<float>(const NvOpDev &op) { return op.cmat32; }
+
+// ------------------------------------------------------------------------
+// mode digits and the diagonal part
+// ------------------------------------------------------------------------
+// Writes the digits n_k of r into dig[k*kBlock + tid]; returns E(r).
+__device__ __forceinline__ double digits_energy(const NvOpDev &op, int r, uint8_t *dig, int tid) {
+    unsigned rr = (unsigned)r;
+    double e = 0.0;
+    for (int k = op.M - 1; k >= 0; --k) {
+        const unsigned nf = (unsigned)op.nfunc[k];
+        const unsigned q = rr / nf;
+        const unsigned d = rr - q * nf;
+        rr = q;
+        dig[k * kBlock + tid] = (uint8_t)d;
+        e += ((double)d + 0.5) * op.omega[k];
+    }
+    return e;
+}
+
+// out[s] = E v[s] + sum_s' D_ss'(r) v[s'] in FP64 (Source/top.f90:712-756).
+template <int NS>
+__device__ __forceinline__ void diag_apply(const NvOpDev &op, double E, const uint8_t *dig, int tid,
+                                           const double (&v)[NS], double (&out)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double a = E * v[s];
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) a += op.dconst[s * NS + s2] * v[s2];
+        out[s] = a;
+    }
+    for (int t = 0; t < op.ndterms; ++t) {
+        const NvDiagTerm dt = op.dterms[t];
+        double h = 1.0;
+#pragma unroll
+        for (int j = 0; j < NV_MAXF; ++j)
+            if (j < dt.nf) h *= op.ftab64[dt.code[j] * op.nmax + dig[dt.mode[j] * kBlock + tid]];
+        const double *C = op.dcmat + (size_t)t * NS * NS;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double a = 0.0;
+#pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2) a += C[s * NS + s2] * v[s2];
+            out[s] += h * a;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// H*v: off-diagonal (in vibrational space) terms, Source/top.f90:758-808.
+// acc[s] = sum_t sum_s' C_t[s][s'] sum_{steps} prod_j f_j(n) x[s'][r+stride]
+// ------------------------------------------------------------------------
+template <int NS, typename TIn, typename TAcc>
+__device__ __forceinline__ void offdiag_accumulate(const NvOpDev &op, const TIn *__restrict__ x, int r,
+                                                   const uint8_t *dig, const TAcc *ftab, int tid,
+                                                   TAcc (&acc)[NS]) {
+    const TAcc *__restrict__ cm = cmat_of<TAcc>(op);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] = (TAcc)0;
+    for (int t = 0; t < op.nterms; ++t) {
+        const NvTerm tm = op.terms[t];
+        TAcc w[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) w[s] = (TAcc)0;
+        for (int q = 0; q < tm.nsteps; ++q) {
+            const NvStep st = op.steps[tm.first + q];
+            TAcc h = ftab[st.code[0] * op.nmax + dig[st.mode[0] * kBlock + tid]];
+#pragma unroll
+            for (int j = 1; j < NV_MAXF; ++j)
+                if (j < st.nf) h *= ftab[st.code[j] * op.nmax + dig[st.mode[j] * kBlock + tid]];
+            if (h != (TAcc)0) {
+                const int idx = r + st.stride;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) w[s] += h * load_acc<TIn, TAcc>(x + (size_t)s * op.D + idx);
+            }
+        }
+        const TAcc *C = cm + (size_t)t * NS * NS;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            TAcc a = acc[s];
+#pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2) a += C[s * NS + s2] * w[s2];
+            acc[s] = a;
+        }
+    }
+}
+
+template <typename TAcc>
+__device__ __forceinline__ TAcc *stage_tables(const NvOpDev &op, uint8_t *&dig) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    TAcc *ftab = reinterpret_cast<TAcc *>(lds_raw);
+    const int nt = op.ncodes * op.nmax;
+    const TAcc *g = ftab_of<TAcc>(op);
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) ftab[i] = g[i];
+    size_t off = ((size_t)nt * sizeof(TAcc) + 15) & ~(size_t)15;
+    dig = lds_raw + off;
+    return ftab;
+}
+
+// Standalone y = H x (x, y double).  TAcc selects the precision of the
+// off-diagonal part; the diagonal part is always FP64.
+template <int NS, typename TAcc>
+__global__ __launch_bounds__(kBlock) void hv_full_kernel(NvOpDev op, const double *__restrict__ x,
+                                                          double *__restrict__ y) {
+    uint8_t *dig;
+    const TAcc *ftab = stage_tables<TAcc>(op, dig);
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x * kBlock + tid;
+    const bool live = r < op.D;
+    double E = 0.0;
+    if (live) E = digits_energy(op, r, dig, tid);
+    __syncthreads();
+    if (!live) return;
+    TAcc acc[NS];
+    offdiag_accumulate<NS, double, TAcc>(op, x, r, dig, ftab, tid, acc);
+    double v[NS], dv[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) v[s] = x[(size_t)s * op.D + r];
+    diag_apply<NS>(op, E, dig, tid, v, dv);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) y[(size_t)s * op.D + r] = dv[s] + (double)acc[s];
+}
+
+__device__ __forceinline__ int64_t slot_of(int64_t i, int64_t nslots) { return i % nslots; }
+
+// Iteration H*v: y = offdiag(H) q_i in TAcc, alpha(i) = q_i . (H q_i) in FP64.
+template <int NS, typename TIn, typename TAcc>
+__global__ __launch_bounds__(kBlock) void hv_iter_kernel(NvOpDev op, LzState st) {
+    uint8_t *dig;
+    const TAcc *ftab = stage_tables<TAcc>(op, dig);
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x * kBlock + tid;
+    const bool live = r < op.D;
+    const int64_t it = *st.iter;
+    const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;
+    const TIn *x = st.vlo ? reinterpret_cast<const TIn *>(st.vlo) : reinterpret_cast<const TIn *>(q);
+    double E = 0.0;
+    if (live) E = digits_energy(op, r, dig, tid);
+    __syncthreads();
+    double part = 0.0;
+    if (live) {
+        TAcc acc[NS];
+        offdiag_accumulate<NS, TIn, TAcc>(op, x, r, dig, ftab, tid, acc);
+        double v[NS], dv[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
+        diag_apply<NS>(op, E, dig, tid, v, dv);
+        TAcc *y = reinterpret_cast<TAcc *>(st.y);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            y[(size_t)s * op.D + r] = acc[s];
+            part += v[s] * (dv[s] + (double)acc[s]);
+        }
+    }
+    double tot;
+    if (last_block_total(part, st.partials, st.counters + kCntAlpha, &tot))
+        if (tid == 0) st.alpha[it] = tot;
+}
+
+// lanczos_step (Source/top.f90:820-842): r = (H q_i - alpha q_i) - beta(i-1) q_{i-1}
+// stored in slot i+1, beta(i) = |r|.
+template <int NS, typename TAcc>
+__global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzState st) {
+    __shared__ uint8_t dig[NV_MAXM * kBlock];
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x * kBlock + tid;
+    const int64_t it = *st.iter;
+    const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;
+    const double *qm = st.V + slot_of(it - 1, st.nslots) * (int64_t)op.N;
+    double *res = st.V + slot_of(it + 1, st.nslots) * (int64_t)op.N;
+    const double a = st.alpha[it];
+    const double b = it > 1 ? st.beta[it - 1] : 0.0;
+    double part = 0.0;
+    if (r < op.D) {
+        const double E = digits_energy(op, r, dig, tid);
+        double v[NS], dv[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
+        diag_apply<NS>(op, E, dig, tid, v, dv);
+        const TAcc *y = reinterpret_cast<const TAcc *>(st.y);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const size_t o = (size_t)s * op.D + r;
+            double w = dv[s] + (double)y[o];
+            w = w - a * v[s];
+            if (it > 1) w = w - b * qm[o];
+            res[o] = w;
+            part += w * w;
+        }
+    }
+    double tot;
+    if (last_block_total(part, st.partials, st.counters + kCntBeta, &tot))
+        if (tid == 0) st.beta[it] = sqrt(tot);
+}
+
+// update_lanczos (Source/top.f90:646-670): q_i = r / beta(i-1) in place, and
+// the copy of q_i in the H*v input precision.
+template <typename TIn>
+__global__ __launch_bounds__(kBlock) void normalize_kernel(int64_t N, LzState st) {
+    const int64_t it = *st.iter;
+    double *q = st.V + slot_of(it, st.nslots) * N;
+    const double sc = 1.0 / st.beta[it - 1];
+    TIn *lo = reinterpret_cast<TIn *>(st.vlo);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock) {
+        const double v = q[i] * sc;
+        q[i] = v;
+        if (lo) lo[i] = from_d<TIn>(v);
+    }
+}
+
+__global__ void set_iter_kernel(int64_t *it, int64_t v, int inc) {
+    if (threadIdx.x == 0) *it = inc ? *it + 1 : v;
+}
+
+__global__ __launch_bounds__(kBlock) void fill_start_kernel(double *v, int64_t D, int ns, const double *vals) {
+    const int64_t N = D * ns;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock) {
+        const int64_t s = i / D, r = i - s * D;
+        v[i] = r == 0 ? vals[s] : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void norm_kernel(const double *v, int64_t n, double *out, double *partials,
+                                                      unsigned *counter) {
+    double p = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        p += v[i] * v[i];
+    double tot;
+    if (last_block_total(p, partials, counter, &tot))
+        if (threadIdx.x == 0) *out = sqrt(tot);
+}
+
+// ------------------------------------------------------------------------
+// host launchers for the above
+// ------------------------------------------------------------------------
+size_t hv_lds_bytes(const NvOpDev &op, int prec) {
+    const size_t el = prec == 0 ? sizeof(double) : sizeof(float);
+    size_t off = ((size_t)op.ncodes * op.nmax * el + 15) & ~(size_t)15;
+    return off + (size_t)op.M * kBlock;
+}
+
+int hv_blocks(const NvOpDev &op) { return (op.D + kBlock - 1) / kBlock; }
+
+static int grid_stride_blocks(int64_t n) {
+    int64_t b = (n + kBlock - 1) / kBlock;
+    return (int)(b < 8192 ? b : 8192);
+}
+
+template <int NS>
+static hipError_t hv_full_ns(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s) {
+    const size_t lds = hv_lds_bytes(op, prec);
+    if (prec == 0)
+        hipLaunchKernelGGL((hv_full_kernel<NS, double>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, x, y);
+    else
+        hipLaunchKernelGGL((hv_full_kernel<NS, float>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, x, y);
+    return hipGetLastError();
+}
+
+template <int NS>
+static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    const size_t lds = hv_lds_bytes(op, prec);
+    if (prec == 0)
+        hipLaunchKernelGGL((hv_iter_kernel<NS, double, double>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
+    else if (prec == 1)
+        hipLaunchKernelGGL((hv_iter_kernel<NS, float, float>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
+    else
+        hipLaunchKernelGGL((hv_iter_kernel<NS, __hip_bfloat16, float>), dim3(hv_blocks(op)), dim3(kBlock), lds, s,
+                           op, st);
+    return hipGetLastError();
+}
+
+template <int NS>
+static hipError_t step_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    if (prec == 0)
+        hipLaunchKernelGGL((lanczos_step_kernel<NS, double>), dim3(hv_blocks(op)), dim3(kBlock), 0, s, op, st);
+    else
+        hipLaunchKernelGGL((lanczos_step_kernel<NS, float>), dim3(hv_blocks(op)), dim3(kBlock), 0, s, op, st);
+    return hipGetLastError();
+}
+
+#define NV_DISPATCH_NS(fn, ...)                    \
+    switch (op.ns) {                               \
+    case 1: return fn<1>(__VA_ARGS__);             \
+    case 2: return fn<2>(__VA_ARGS__);             \
+    case 3: return fn<3>(__VA_ARGS__);             \
+    case 4: return fn<4>(__VA_ARGS__);             \
+    case 5: return fn<5>(__VA_ARGS__);             \
+    case 6: return fn<6>(__VA_ARGS__);             \
+    case 7: return fn<7>(__VA_ARGS__);             \
+    case 8: return fn<8>(__VA_ARGS__);             \
+    case 9: return fn<9>(__VA_ARGS__);             \
+    case 10: return fn<10>(__VA_ARGS__);           \
+    default: return hipErrorInvalidValue;          \
+    }
+
+hipError_t launch_hv_full(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s) {
+    NV_DISPATCH_NS(hv_full_ns, op, prec, x, y, s)
+}
+
+hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    NV_DISPATCH_NS(hv_iter_ns, op, prec, st, s)
+}
+
+hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    NV_DISPATCH_NS(step_ns, op, prec, st, s)
+}
+
+hipError_t launch_normalize(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    const int g = grid_stride_blocks(op.N);
+    if (prec == 0 || st.vlo == nullptr)
+        hipLaunchKernelGGL((normalize_kernel<double>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N, st);
+    else if (prec == 1)
+        hipLaunchKernelGGL((normalize_kernel<float>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N, st);
+    else
+        hipLaunchKernelGGL((normalize_kernel<__hip_bfloat16>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_iter(int64_t *d_iter, int64_t value, int increment, hipStream_t s) {
+    hipLaunchKernelGGL(set_iter_kernel, dim3(1), dim3(64), 0, s, d_iter, value, increment);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_start(double *v, int64_t D, int ns, const double *vals_dev, hipStream_t s) {
+    hipLaunchKernelGGL(fill_start_kernel, dim3(grid_stride_blocks(D * ns)), dim3(kBlock), 0, s, v, D, ns, vals_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_norm_to(const double *v, int64_t n, double *out, double *partials, unsigned *counter,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(norm_kernel, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, v, n, out, partials, counter);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Tridiagonal eigensolve (restates GIVENS, Source/yarkony.f90:91-543, for a
+// tridiagonal input: the Householder stage and the back transformation are
+// identities there).  The matrix arrives divided by its Frobenius norm.
+// ------------------------------------------------------------------------
+#define NV_EPS 1.1102230246251565e-16          /* 2^-53, determine_epsilon */
+#define NV_HUGE 1.7976931348623157e308
+#define NV_DEL1 (NV_EPS / 100.0)
+#define NV_DELTA (NV_EPS * NV_EPS * 100.0)
+#define NV_DELBIG (NV_HUGE * NV_DELTA / 1000.0)
+#define NV_THETA1 (1000.0 / NV_HUGE)
+#define NV_TOLER (NV_EPS * 100.0)
+
+// Number of eigenvalues below x: negative pivots of the LDL^T of T - x,
+// with a pivot below THETA1 in magnitude treated as in Source/yarkony.f90:360-389.
+__device__ int64_t sturm_below(const double *d, const double *e2, int64_t n, double x) {
+    int64_t nonneg = 0;
+    int64_t j = 0;
+    double f = d[0] - x;
+    while (true) {
+        if (fabs(f) < NV_THETA1) {
+            j += 2;
+            nonneg += 1;
+            if (j >= n) break;
+            f = d[j] - x;
+            continue;
+        }
+        if (f >= 0.0) nonneg += 1;
+        j += 1;
+        if (j >= n) break;
+        f = d[j] - x - e2[j - 1] / f;
+    }
+    return n - nonneg;
+}
+
+__global__ void tri_bisect_kernel(int64_t n, const double *d, const double *e2, double *root) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double lo = -1.0, hi = 1.0;
+    for (int guard = 0; guard < 2200; ++guard) {
+        const double mid = (lo + hi) * 0.5;
+        if (mid == lo || mid == hi) break;
+        if (sturm_below(d, e2, n, mid) >= i + 1) hi = mid;
+        else lo = mid;
+    }
+    root[i] = hi;
+}
+
+// Back substitution through the pivoted factors of one root (Source/yarkony.f90:790-780).
+// Returns the last pivot used (the value the forward sweep of the reference
+// carries in TEMP).
+__device__ double tri_backsolve(int64_t n, int64_t i, const double *U2, const double *U3, const double *U4,
+                                double *v) {
+    double t = 1.0;
+    for (int64_t l = n - 1; l >= 0; --l) {
+        while (true) {
+            const double v1 = l + 1 < n ? v[(l + 1) * n + i] : 0.0;
+            const double v2 = l + 2 < n ? v[(l + 2) * n + i] : 0.0;
+            const double e = v[l * n + i] - v1 * U3[l * n + i] - v2 * U4[l * n + i];
+            if (fabs(e) > NV_DELBIG) {
+                for (int64_t k = 0; k < n; ++k) v[k * n + i] /= NV_DELBIG;
+                continue;
+            }
+            t = U2[l * n + i];
+            if (fabs(t) < NV_DELTA) t = NV_DELTA;
+            v[l * n + i] = e / t;
+            break;
+        }
+    }
+    return t;
+}
+
+// Inverse iteration for root i: factor T - root (partial pivoting), start
+// from ones (or the reference's multiplicative generator inside a cluster),
+// back solve, forward sweep, back solve.  Vectors are stored component-major:
+// vec[j*n + i].
+__global__ void tri_inverse_kernel(int64_t n, const double *d, const double *e, const double *root,
+                                   const int32_t *ia, const int64_t *rank, double *lu, double *vec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t nn = n * n;
+    double *U2 = lu, *U3 = lu + nn, *U4 = lu + 2 * nn, *U5 = lu + 3 * nn;
+    const double ar = root[i];
+    double el1 = d[0] - ar;
+    double el2 = n > 1 ? e[0] : 0.0;
+    for (int64_t j = 0; j + 1 < n; ++j) {
+        const double bj = e[j];
+        const double bj1 = j + 2 < n ? e[j + 1] : 0.0;
+        double t;
+        if (fabs(el1) <= fabs(bj)) {
+            U2[j * n + i] = bj;
+            U3[j * n + i] = d[j + 1] - ar;
+            U4[j * n + i] = bj1;
+            t = 1.0;
+            if (fabs(bj) > NV_THETA1) t = el1 / bj;
+            el1 = el2 - t * (d[j + 1] - ar);
+            el2 = -t * bj1;
+        } else {
+            U2[j * n + i] = el1;
+            U3[j * n + i] = el2;
+            U4[j * n + i] = 0.0;
+            t = bj / el1;
+            el1 = d[j + 1] - ar - t * el2;
+            el2 = bj1;
+        }
+        U5[j * n + i] = t;
+    }
+    U2[(n - 1) * n + i] = el1;
+    U3[(n - 1) * n + i] = 0.0;
+    U4[(n - 1) * n + i] = 0.0;
+    if (n >= 2) U4[(n - 2) * n + i] = 0.0;
+    // start vector
+    if (ia[i] == 0) {
+        for (int64_t j = 0; j < n; ++j) vec[j * n + i] = 1.0;
+    } else {
+        // state of x <- 4099 x mod 2^23 after n*rank[i] draws from 2^23-3
+        const uint64_t m = 1ull << 23;
+        uint64_t a = 4099, x = m - 3;
+        uint64_t k = (uint64_t)n * (uint64_t)rank[i];
+        uint64_t p = a % m, acc = 1;
+        while (k) {
+            if (k & 1) acc = (acc * p) % m;
+            p = (p * p) % m;
+            k >>= 1;
+        }
+        x = (x * acc) % m;
+        for (int64_t j = 0; j < n; ++j) {
+            x = (4099ull * x) % m;
+            vec[j * n + i] = (double)x / (double)(m / 2) - 1.0;
+        }
+    }
+    const double t1 = tri_backsolve(n, i, U2, U3, U4, vec);
+    // forward sweep (Source/yarkony.f90:890-830); a swapped row uses the last
+    // pivot of the back substitution as multiplier, as the reference does.
+    double el = vec[0 * n + i];
+    for (int64_t j = 0; j + 1 < n; ++j) {
+        const double nxt = vec[(j + 1) * n + i];
+        if (U2[j * n + i] == e[j]) {
+            vec[j * n + i] = nxt;
+            el = el - nxt * t1;
+        } else {
+            vec[j * n + i] = el;
+            el = nxt - el * U5[j * n + i];
+        }
+    }
+    vec[(n - 1) * n + i] = el;
+    tri_backsolve(n, i, U2, U3, U4, vec);
+}
+
+__device__ void tri_normalize(int64_t n, int64_t i, double *vec) {
+    double mx = 0.0;
+    for (int64_t j = 0; j < n; ++j) mx = fmax(fabs(vec[j * n + i]), mx);
+    double s = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+        const double q = vec[j * n + i] / mx;
+        s += q * q;
+    }
+    const double t = 1.0 / (sqrt(s) * mx);
+    for (int64_t j = 0; j < n; ++j) {
+        double v = vec[j * n + i] * t;
+        if (fabs(v) < NV_DEL1) v = 0.0;
+        vec[j * n + i] = v;
+    }
+}
+
+// Gram-Schmidt inside clusters of near-equal roots (Source/yarkony.f90:800-870)
+// and normalisation (:900-920).  A cluster is handled by the thread of its
+// first root, member by member.
+__global__ void tri_finish_kernel(int64_t n, const int32_t *ia, double *vec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || ia[i] != 0) return;
+    tri_normalize(n, i, vec);
+    for (int64_t m = i + 1; m < n && ia[m] > 0; ++m) {
+        for (int64_t j1 = 1; j1 <= ia[m]; ++j1) {
+            const int64_t k = m - j1;
+            double t = 0.0;
+            for (int64_t j = 0; j < n; ++j) t += vec[j * n + m] * vec[j * n + k];
+            for (int64_t j = 0; j < n; ++j) vec[j * n + m] -= t * vec[j * n + k];
+        }
+        tri_normalize(n, m, vec);
+    }
+}
+
+__global__ void tri_cluster_kernel(int64_t n, const double *root, int32_t *ia, int64_t *rank) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int32_t c = 0;
+    int64_t r = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i > 0 && fabs(root[i - 1] - root[i]) < NV_TOLER) c += 1;
+        else c = 0;
+        ia[i] = c;
+        rank[i] = r;
+        if (c != 0) r += 1;
+    }
+}
+
+hipError_t launch_tridiag(int64_t n, const TriWork &w, double anorm, hipStream_t s) {
+    (void)anorm;
+    const int tb = 64;
+    const int g = (int)((n + tb - 1) / tb);
+    hipLaunchKernelGGL(tri_bisect_kernel, dim3(g), dim3(tb), 0, s, n, w.d, w.e2, w.root);
+    hipLaunchKernelGGL(tri_cluster_kernel, dim3(1), dim3(64), 0, s, n, w.root, w.ia, w.rank);
+    hipLaunchKernelGGL(tri_inverse_kernel, dim3(g), dim3(tb), 0, s, n, w.d, w.e, w.root, w.ia, w.rank, w.lu, w.vec);
+    hipLaunchKernelGGL(tri_finish_kernel, dim3(g), dim3(tb), 0, s, n, w.ia, w.vec);
+    return hipGetLastError();
+}
+
+}  // namespace nv

@@ -1,0 +1,62 @@
+// Host launchers of the HIP kernels (implemented in nv_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nv_device.hpp"
+
+namespace nv {
+
+constexpr int kBlock = 256;
+
+// Device-side state of one Lanczos run.  Vector q_j lives in slot j % nslots
+// of V (nslots = niter+2 when the Krylov basis is kept, else 3); the slot of
+// iteration i+1 holds the unnormalised residual until update_lanczos(i+1).
+struct LzState {
+    double *V = nullptr;
+    int64_t nslots = 0;
+    void *vlo = nullptr;         // q_i in the H*v input precision (F32/BF16)
+    void *y = nullptr;           // off-diagonal part of H q_i (accumulation type)
+    double *alpha = nullptr;     // [niter+2], alpha[i]
+    double *beta = nullptr;      // [niter+2], beta[i], beta[0] = |start|
+    double *partials = nullptr;  // per-block partial sums
+    unsigned *counters = nullptr;
+    const int64_t *iter = nullptr;  // current iteration (device)
+};
+
+// counters[] slots
+enum { kCntAlpha = 0, kCntBeta = 1, kCntPro = 2, kCntChk = 3, kCntMisc = 4, kNumCounters = 8 };
+
+size_t hv_lds_bytes(const NvOpDev &op, int prec);
+int hv_blocks(const NvOpDev &op);
+
+// y = H x, x and y double, full operator (standalone H*v).
+hipError_t launch_hv_full(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s);
+// Converts q_i to the input precision of H*v.
+hipError_t launch_set_iter(int64_t *d_iter, int64_t value, int increment, hipStream_t s);
+// update_lanczos: q_i = r_{i-1} / beta(i-1), plus the low-precision copy.
+hipError_t launch_normalize(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
+// Hv: off-diagonal part of H q_i and alpha(i) = q_i . H q_i.
+hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
+// lanczos_step: r_i = H q_i - alpha q_i - beta(i-1) q_{i-1}, beta(i) = |r_i|.
+hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
+// Writes the start vector: zero except vib index 0 of each state, from vals[ns].
+hipError_t launch_fill_start(double *v, int64_t D, int ns, const double *vals_dev, hipStream_t s);
+// beta[0] = |v| over n doubles.
+hipError_t launch_norm_to(const double *v, int64_t n, double *out, double *partials, unsigned *counter,
+                          hipStream_t s);
+
+// ---- tridiagonal eigensolve ----
+struct TriWork {
+    double *d = nullptr;     // [n] normalised diagonal
+    double *e = nullptr;     // [n] normalised off-diagonal, e[j] couples j, j+1
+    double *e2 = nullptr;    // [n] squares
+    double *root = nullptr;  // [n]
+    double *vec = nullptr;   // [n][n], vec[i*n + j] = component j of root i
+    double *lu = nullptr;    // [5][n][n] per-root elimination rows, interleaved by root
+    int32_t *ia = nullptr;   // [n] position inside a cluster of near-equal roots
+    int64_t *rank = nullptr; // [n] number of cluster members (ia>0) before root i
+};
+hipError_t launch_tridiag(int64_t n, const TriWork &w, double anorm, hipStream_t s);
+
+}  // namespace nv

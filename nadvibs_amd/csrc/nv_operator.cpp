@@ -1,0 +1,314 @@
+// Operator construction from the raw nadvibs.in data.  See nv_operator.hpp.
+#include "nv_operator.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <tuple>
+
+namespace nv {
+
+// Matrix element <m|q^p|m+delta>, q = (a + a^dagger)/sqrt(2), for the
+// (p, delta) pairs the steps can reach.  The arithmetic form of every
+// expression is the one of Source/function.f90:198-246 (m there is the
+// 0-based quantum of the lower function) so that the table rounds the same
+// way.  With NV_QUIRK_Q3ELEM the p=3, delta=1 element keeps the reference's
+// missing factor 3 (Source/function.f90:225).  Powers above 4 are zero in the
+// reference (case default); the exact mode evaluates them by ladder algebra.
+static double ladder_power_element(int p, int64_t m, int delta) {
+    // <m|q^p|m+delta> by repeated application of q on a basis big enough to
+    // hold every intermediate state.
+    const int64_t L = m + delta + p + 2;
+    std::vector<double> v(L, 0.0), w(L, 0.0);
+    v[m + delta] = 1.0;  // |m+delta>
+    for (int s = 0; s < p; ++s) {
+        std::fill(w.begin(), w.end(), 0.0);
+        for (int64_t n = 0; n < L; ++n) {
+            if (v[n] == 0.0) continue;
+            if (n + 1 < L) w[n + 1] += v[n] * std::sqrt((n + 1) / 2.0);
+            if (n >= 1) w[n - 1] += v[n] * std::sqrt(n / 2.0);
+        }
+        v.swap(w);
+    }
+    return v[m];
+}
+
+double hoelem(int p, int64_t mi, int delta, int32_t quirks) {
+    const double m = (double)mi;
+    switch (p) {
+    case 1:
+        if (delta == 1) return std::sqrt((m + 1) / 2);
+        return 0.0;
+    case 2:
+        if (delta == 2) return std::sqrt((m + 1) * (m + 2) / 4);
+        if (delta == 0) return (2 * m + 1) / 2;
+        return 0.0;
+    case 3:
+        if (delta == 3) return std::sqrt((m + 1) * (m + 2) * (m + 3) / 8);
+        if (delta == 1) {
+            const double e = std::sqrt(((m + 1) * (m + 1) * (m + 1)) / 8);
+            return (quirks & NV_QUIRK_Q3ELEM) ? e : 3.0 * e;
+        }
+        return 0.0;
+    case 4:
+        if (delta == 4) return std::sqrt((m + 1) * (m + 2) * (m + 3) * (m + 4)) / 4;
+        if (delta == 2) return (2 * m + 3) * std::sqrt((m + 1) * (m + 2)) / 2;
+        if (delta == 0) return 3 * (2 * m * m + 2 * m + 1) / 4;
+        return 0.0;
+    default:
+        if (quirks & NV_QUIRK_Q3ELEM) return 0.0;  // reference: powers > 4 vanish
+        return ladder_power_element(p, mi, delta);
+    }
+}
+
+int64_t noterms(int64_t nmodes, int64_t p) {
+    // C(nmodes+p-1, p) by exact integer products (Source/function.f90:253-258).
+    int64_t r = 1;
+    for (int64_t i = 1; i <= p; ++i) r = r * (nmodes - 1 + i) / i;
+    return r;
+}
+
+std::vector<std::vector<int>> monomials(int64_t nmodes, int64_t p) {
+    // nadvibs.in lists a monomial as a non-increasing mode tuple with the
+    // first entry running fastest; reversed, these are the non-decreasing
+    // tuples in lexicographic order, generated here directly.
+    std::vector<std::vector<int>> out;
+    std::vector<int> b(p, 0);
+    while (true) {
+        out.emplace_back(b.rbegin(), b.rend());
+        int i = (int)p - 1;
+        while (i >= 0 && b[i] == nmodes - 1) --i;
+        if (i < 0) break;
+        int v = b[i] + 1;
+        for (int j = i; j < p; ++j) b[j] = v;
+    }
+    return out;
+}
+
+static int64_t ipow_prod(const std::vector<int64_t> &v) {
+    int64_t r = 1;
+    for (auto x : v) r *= x;
+    return r;
+}
+
+bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
+    if (pr.nstates < 1 || pr.nstates > NV_MAXS) { err = "nstates must be in 1..10"; return false; }
+    if (pr.nmodes < 1 || pr.nmodes > NV_MAXM) { err = "nmodes must be in 1..64"; return false; }
+    if (pr.ordr < 1 || pr.ordr > NV_MAXF) { err = "ordr must be in 1..4"; return false; }
+    if (!pr.nfunc || !pr.aomega || !pr.concoef || !pr.potterms) { err = "null problem array"; return false; }
+    op.ns = pr.nstates; op.M = pr.nmodes; op.ordr = pr.ordr; op.quirks = pr.quirks;
+    op.nstblks = op.ns * (op.ns + 1) / 2;
+    op.nfunc.assign(pr.nfunc, pr.nfunc + op.M);
+    op.omega.assign(pr.aomega, pr.aomega + op.M);
+    op.concoef.assign(pr.concoef, pr.concoef + op.nstblks);
+    op.nmax = 1;
+    for (int64_t k = 0; k < op.M; ++k) {
+        if (op.nfunc[k] < 1 || op.nfunc[k] > 255) { err = "basis sizes must be in 1..255"; return false; }
+        if (!(op.omega[k] > 0)) { err = "frequencies must be positive"; return false; }
+        op.nmax = std::max(op.nmax, op.nfunc[k]);
+    }
+    op.D = ipow_prod(op.nfunc);
+    op.N = op.D * op.ns;
+    if (op.N >= (int64_t(1) << 31)) { err = "nstates*dimen must be below 2^31"; return false; }
+    // shft_k = prod_{j>k} nfunc_j: last mode fastest (Source/top.f90:379-386)
+    op.shft.assign(op.M, 1);
+    for (int64_t k = op.M - 2; k >= 0; --k) op.shft[k] = op.shft[k + 1] * op.nfunc[k + 1];
+
+    // ---- term numbering (Source/top.f90:150-201) ----
+    const int64_t ntmax = noterms(op.M, op.ordr);
+    std::vector<std::vector<Term>> bycnt(op.ordr + 1);
+    for (int64_t b = 0; b < op.nstblks; ++b) {
+        for (int64_t p = 1; p <= op.ordr; ++p) {
+            auto mons = monomials(op.M, p);
+            for (size_t k = 0; k < mons.size(); ++k) {
+                double c = pr.potterms[k + (p - 1) * ntmax + b * ntmax * op.ordr];
+                if (!(std::fabs(c) >= pr.ztoler)) continue;
+                // unique modes in first-appearance order and their powers
+                int um[NV_MAXF], up[NV_MAXF], cnt = 0;
+                for (int v : mons[k]) {
+                    int f = -1;
+                    for (int j = 0; j < cnt; ++j) if (um[j] == v) f = j;
+                    if (f < 0) { um[cnt] = v; up[cnt] = 1; ++cnt; } else up[f]++;
+                }
+                auto &list = bycnt[cnt];
+                Term *t = nullptr;
+                for (auto &e : list) {
+                    bool same = true;
+                    for (int j = 0; j < cnt; ++j) same = same && e.mode[j] == um[j] && e.pow[j] == up[j];
+                    if (same) { t = &e; break; }
+                }
+                if (!t) {
+                    list.emplace_back();
+                    t = &list.back();
+                    t->cnt = cnt;
+                    for (int j = 0; j < cnt; ++j) { t->mode[j] = um[j]; t->pow[j] = up[j]; }
+                }
+                t->blocks.push_back((int)b);
+                t->raw.push_back(c);
+            }
+        }
+    }
+    op.nztrms.assign(op.ordr, 0);
+    for (int64_t c = 1; c <= op.ordr; ++c) {
+        op.nztrms[c - 1] = (int64_t)bycnt[c].size();
+        for (auto &t : bycnt[c]) op.terms.push_back(t);
+    }
+
+    // block index <-> state pair (cpindx, Source/top.f90:80-85): b = i(i-1)/2+j-1
+    std::vector<int> bi(op.nstblks), bj(op.nstblks);
+    for (int i = 1; i <= op.ns; ++i)
+        for (int j = 1; j <= i; ++j) { bi[i * (i - 1) / 2 + j - 1] = i - 1; bj[i * (i - 1) / 2 + j - 1] = j - 1; }
+
+    // ---- dval scaling, steps, codes ----
+    std::map<std::tuple<int, int, int>, int> codes;  // (mode, pow, delta) -> row
+    auto code_of = [&](int mode, int p, int delta) {
+        auto key = std::make_tuple(mode, p, delta);
+        auto it = codes.find(key);
+        if (it != codes.end()) return it->second;
+        int c = (int)codes.size();
+        codes[key] = c;
+        return c;
+    };
+    const bool lower = (op.quirks & NV_QUIRK_LOWERDIAG) != 0;
+    const int ns = (int)op.ns;
+    op.ntcom.assign(op.ordr * op.nstblks, 0);
+    std::vector<char> active(op.nstblks, 0);
+    for (int64_t b = 0; b < op.nstblks; ++b) active[b] = std::fabs(op.concoef[b]) > 1e-16;
+    op.nnz_fma = 0;
+    for (auto &t : op.terms) {
+        t.dval = 1.0;
+        for (int j = 0; j < t.cnt; ++j) {
+            double w = op.omega[t.mode[j]], wp = 1.0;
+            for (int e = 0; e < t.pow[j]; ++e) wp *= w;
+            t.dval = t.dval / std::sqrt(wp);
+        }
+        // smax (Source/secondary.f90:29-39)
+        t.ndir = 1;
+        bool even = true;
+        for (int j = 0; j < t.cnt; ++j) {
+            int s = t.pow[j];
+            while (s > op.nfunc[t.mode[j]] - 1) s -= 2;
+            t.smax[j] = s;
+            if (s < 0) t.ndir = 0;
+            else t.ndir *= (s + 1);
+            if (s % 2 != 0) even = false;
+        }
+        t.diag = even && t.ndir > 0;
+        if (t.diag)
+            for (int b : t.blocks) active[b] = 1;
+    }
+    // coupling matrices, steps and the term-count check
+    int64_t trm_rows = 0;
+    for (auto &t : op.terms) {
+        std::vector<double> C(ns * ns, 0.0);
+        int npairs = 0;
+        for (size_t q = 0; q < t.blocks.size(); ++q) {
+            int b = t.blocks[q];
+            double c = t.raw[q] * t.dval;
+            C[bi[b] * ns + bj[b]] += c; ++npairs;
+            if (bi[b] != bj[b]) { C[bj[b] * ns + bi[b]] += c; ++npairs; }
+        }
+        int64_t fixed = op.D;
+        for (int j = 0; j < t.cnt; ++j) fixed /= op.nfunc[t.mode[j]];
+        // enumerate Delta vectors, slot 0 slowest (Source/secondary.f90:93-106)
+        NvTerm dt;
+        dt.first = (int32_t)op.dev_steps.size();
+        dt.nsteps = 0;
+        if (t.ndir > 0) {
+            int d[NV_MAXF];
+            for (int j = 0; j < t.cnt; ++j) d[j] = -t.smax[j];
+            while (true) {
+                int64_t rows = fixed, stride = 0;
+                bool zero = true;
+                for (int j = 0; j < t.cnt; ++j) {
+                    rows *= op.nfunc[t.mode[j]] - std::abs(d[j]);
+                    stride += (int64_t)d[j] * op.shft[t.mode[j]];
+                    zero = zero && d[j] == 0;
+                }
+                for (int b : t.blocks) op.ntcom[(t.cnt - 1) * op.nstblks + b] += rows;
+                if (!zero) {
+                    NvStep s;
+                    s.stride = (int32_t)stride;
+                    s.nf = t.cnt;
+                    for (int j = 0; j < NV_MAXF; ++j) { s.mode[j] = 0; s.code[j] = 0; }
+                    for (int j = 0; j < t.cnt; ++j) {
+                        s.mode[j] = t.mode[j];
+                        s.code[j] = code_of(t.mode[j], t.pow[j], d[j]);
+                    }
+                    op.dev_steps.push_back(s);
+                    dt.nsteps++;
+                    op.nnz_fma += rows * npairs;
+                }
+                int j = t.cnt - 1;
+                while (j >= 0 && d[j] >= t.smax[j]) { d[j] = -t.smax[j]; --j; }
+                if (j < 0) break;
+                d[j] += 2;
+            }
+        }
+        if (dt.nsteps > 0) {
+            op.dev_terms.push_back(dt);
+            op.cmat.insert(op.cmat.end(), C.begin(), C.end());
+        }
+        if (t.diag) {
+            NvDiagTerm dd;
+            dd.nf = t.cnt;
+            for (int j = 0; j < NV_MAXF; ++j) { dd.mode[j] = 0; dd.code[j] = 0; }
+            for (int j = 0; j < t.cnt; ++j) { dd.mode[j] = t.mode[j]; dd.code[j] = code_of(t.mode[j], t.pow[j], 0); }
+            op.dev_dterms.push_back(dd);
+            std::vector<double> Dm(ns * ns, 0.0);
+            for (int s = 0; s < ns; ++s)
+                for (int s2 = 0; s2 < ns; ++s2)
+                    if (s == s2 || !lower || s > s2) Dm[s * ns + s2] = C[s * ns + s2];
+            op.dcmat.insert(op.dcmat.end(), Dm.begin(), Dm.end());
+        }
+    }
+    (void)trm_rows;
+    // constant block matrix, gated by the diagonal-block list (Source/top.f90:528-553)
+    op.dconst.assign(ns * ns, 0.0);
+    int napplied = 0;
+    for (int64_t b = 0; b < op.nstblks; ++b) {
+        int i = bi[b], j = bj[b];
+        if (i == j) { op.dconst[i * ns + i] = op.concoef[b]; continue; }
+        if (!active[b]) continue;
+        op.dconst[i * ns + j] = op.concoef[b];
+        ++napplied;
+        if (!lower) { op.dconst[j * ns + i] = op.concoef[b]; ++napplied; }
+    }
+    op.nnz_fma += op.D * (op.ns + napplied);
+    // factor table ftab[code][n]
+    op.ncodes = (int32_t)codes.size();
+    op.ftab.assign((size_t)op.ncodes * op.nmax, 0.0);
+    for (auto &kv : codes) {
+        int mode = std::get<0>(kv.first), p = std::get<1>(kv.first), delta = std::get<2>(kv.first);
+        int64_t nf = op.nfunc[mode];
+        for (int64_t n = 0; n < nf; ++n) {
+            int64_t n2 = n + delta;
+            if (n2 < 0 || n2 >= nf) continue;
+            op.ftab[(size_t)kv.second * op.nmax + n] = hoelem(p, std::min(n, n2), std::abs(delta), op.quirks);
+        }
+    }
+
+    // ---- start-vector data ----
+    op.neworigin = pr.neworigin;
+    op.statew.assign(op.ns, 1.0);
+    if (pr.statew) op.statew.assign(pr.statew, pr.statew + op.ns);
+    op.istate.assign(op.M, 0);
+    if (pr.istate) op.istate.assign(pr.istate, pr.istate + op.M);
+    if (pr.nirreps >= 1 && pr.npirrep) {
+        op.npirrep.assign(pr.npirrep, pr.npirrep + pr.nirreps);
+        int64_t tot = 0;
+        for (auto v : op.npirrep) tot += v;
+        if (tot != op.M) op.npirrep.assign(1, op.M);  // symmetry disabled (Source/secondary.f90:848-852)
+    } else {
+        op.npirrep.assign(1, op.M);
+    }
+    if (op.neworigin) {
+        if (!pr.bomega || !pr.dvec || !pr.tmat) { err = "neworigin needs bomega, dvec and tmat"; return false; }
+        op.bomega.assign(pr.bomega, pr.bomega + op.M);
+        op.dvec.assign(pr.dvec, pr.dvec + op.M);
+        op.tmat.assign(pr.tmat, pr.tmat + op.M * op.M);
+    }
+    return true;
+}
+
+}  // namespace nv

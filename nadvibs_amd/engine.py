@@ -1,0 +1,242 @@
+"""Python host side of the engine: mirrors the reference's Lanczos driver
+(Source/main.f90:57-78) on top of the C ABI.
+
+    eng = Engine(problem)
+    eng.lanczos_init(niter=300)
+    for i in range(1, 301):
+        eng.update_lanczos(i); eng.Hv(i); eng.lanczos_step(i)
+        if orthog and i > 1: eng.set_omega(i); eng.partial_orthog(i)
+    alpha, beta, _ = eng.coefficients()
+    table = compute_eigenvalues(eng, alpha, beta, bconv)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import NvError, dptr, iptr
+from .inputs import AU2EV, AU2WAVE, Problem
+
+
+class Engine:
+    def __init__(self, prob: Problem, device=0):
+        self.lib = _lib.load()
+        self.prob = prob
+        p = prob
+        self._keep = []
+        nfunc = np.ascontiguousarray(p.nfunc, dtype=np.int64)
+        aomega = np.ascontiguousarray(p.aomega, dtype=np.float64)
+        concoef = np.ascontiguousarray(p.concoef, dtype=np.float64)
+        pot = np.ascontiguousarray(p.potterms, dtype=np.float64)
+        statew = np.ascontiguousarray(p.statew if p.statew is not None else np.ones(p.nstates), dtype=np.float64)
+        npirr = np.ascontiguousarray(p.npirrep or [p.nmodes], dtype=np.int64)
+        istate = np.ascontiguousarray(p.initstate or [0] * p.nmodes, dtype=np.int64)
+        neworigin = bool(p.shiftref)
+        bo = dv = tm = None
+        if neworigin:
+            bo = np.ascontiguousarray(p.bomega, dtype=np.float64)
+            dv = np.ascontiguousarray(p.dvec, dtype=np.float64)
+            tm = np.ascontiguousarray(p.tmat, dtype=np.float64)
+        self._keep += [nfunc, aomega, concoef, pot, statew, npirr, istate, bo, dv, tm]
+        desc = _lib.NvProblem(
+            nstates=p.nstates, nmodes=p.nmodes, ordr=p.ordr, nfunc=iptr(nfunc), aomega=dptr(aomega),
+            concoef=dptr(concoef), potterms=dptr(pot), ztoler=p.ztoler, quirks=p.quirks, neworigin=int(neworigin),
+            statew=dptr(statew), bomega=dptr(bo), dvec=dptr(dv), tmat=dptr(tm), nirreps=len(npirr),
+            npirrep=iptr(npirr), istate=iptr(istate))
+        h = ctypes.c_void_p()
+        rc = self.lib.nv_create(ctypes.byref(desc), device, ctypes.byref(h))
+        if rc != 0:
+            raise NvError(rc, self.lib.nv_last_error(None).decode())
+        self.h = h
+        d, n = ctypes.c_int64(), ctypes.c_int64()
+        self.lib.nv_dims(h, ctypes.byref(d), ctypes.byref(n))
+        self.dimen, self.n = d.value, n.value
+        self.niter = 0
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            raise NvError(rc, self.lib.nv_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- operator -----------------------------------------------------------
+    def term_summary(self):
+        out = np.zeros(self.prob.ordr, dtype=np.int64)
+        self._check(self.lib.nv_term_summary(self.h, iptr(out)))
+        return out
+
+    def term_counts(self):
+        """ntcom(blk, i) of the reference's check, shape [ordr, nstblks]."""
+        out = np.zeros(self.prob.ordr * self.prob.nstblks, dtype=np.int64)
+        self._check(self.lib.nv_term_counts(self.h, iptr(out)))
+        return out.reshape(self.prob.ordr, self.prob.nstblks)
+
+    def hv(self, x, prec="f64"):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        if x.size != self.n:
+            raise ValueError("x has the wrong length")
+        y = np.empty_like(x)
+        self._check(self.lib.nv_hv(self.h, _lib.PREC[prec], dptr(x), dptr(y)))
+        return y
+
+    def hv_device(self, x_ptr, y_ptr, prec="f64", stream=None):
+        self._check(self.lib.nv_hv_device(self.h, _lib.PREC[prec], ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr),
+                                          ctypes.c_void_p(stream) if stream else None))
+
+    def start_vector(self):
+        q = np.empty(self.n)
+        self._check(self.lib.nv_start_vector(self.h, dptr(q)))
+        return q
+
+    # -- Lanczos ---------------------------------------------------------------
+    def lanczos_init(self, niter, reorthog=0, prec="f64", chkorthog=100, seed=12345, noise=1.0):
+        o = _lib.NvLanczosOpts(niter=niter, reorthog=reorthog, prec=_lib.PREC[prec], chkorthog=chkorthog,
+                               seed=seed, noise_scale=noise)
+        self._check(self.lib.nv_lanczos_init(self.h, ctypes.byref(o)))
+        self.niter = niter
+        self.reorthog = reorthog
+
+    def update_lanczos(self, i):
+        self._check(self.lib.nv_update_lanczos(self.h, i))
+
+    def Hv(self, i):
+        self._check(self.lib.nv_hv_iter(self.h, i))
+
+    def lanczos_step(self, i):
+        self._check(self.lib.nv_lanczos_step(self.h, i))
+
+    def set_omega(self, i):
+        self._check(self.lib.nv_set_omega(self.h, i))
+
+    def partial_orthog(self, i):
+        self._check(self.lib.nv_partial_orthog(self.h, i))
+
+    def run(self, iiter=1, niter=None):
+        self._check(self.lib.nv_lanczos_run(self.h, iiter, niter or self.niter))
+
+    def coefficients(self, n=None, omega=False):
+        n = n or self.niter
+        a = np.zeros(n)
+        b = np.zeros(n + 1)
+        o = np.zeros(2 * (n + 1)) if omega else None
+        self._check(self.lib.nv_get_coefficients(self.h, dptr(a), dptr(b), dptr(o)))
+        return a, b, (o.reshape(n + 1, 2).T.copy() if omega else None)
+
+    def pro_stats(self):
+        e, t = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.nv_pro_stats(self.h, ctypes.byref(e), ctypes.byref(t)))
+        return e.value, t.value
+
+    def lanczos_vector(self, j):
+        q = np.empty(self.n)
+        self._check(self.lib.nv_get_lanczos_vector(self.h, j, dptr(q)))
+        return q
+
+    def ritz(self, alpha, beta):
+        """Eigenvalues, intensities, bji and signs of T (alpha, beta(0:n))."""
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        n = alpha.size
+        if beta.size != n + 1:
+            raise ValueError("beta must hold beta(0:n)")
+        ev, it, bj = np.empty(n), np.empty(n), np.empty(n)
+        sg = np.empty(n, dtype=np.int32)
+        self._check(self.lib.nv_ritz(self.h, n, dptr(alpha), dptr(beta), dptr(ev), dptr(it), dptr(bj),
+                                     sg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return ev, it, bj, sg
+
+    def save_vectors(self, path):
+        self._check(self.lib.nv_save_vectors(self.h, os.fsencode(path)))
+
+    def load_vectors(self, path, it, alpha, beta):
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        self._check(self.lib.nv_load_vectors(self.h, os.fsencode(path), it, dptr(alpha), dptr(beta), None))
+
+
+def select_roots(evals, inten, bji, bconv=0):
+    """Roots kept by compute_eigenvalues (Source/top.f90:1066-1086): the
+    lowest root and every later root with bji < 10^-bconv (the merging
+    threshold there is 0, so nothing merges).  Returns 0-based indices."""
+    bjiconv = 10.0 ** (-bconv)
+    keep = [0]
+    for i in range(1, len(evals)):
+        if bji[i] < bjiconv:
+            keep.append(i)
+    return np.array(keep, dtype=np.int64)
+
+
+def _es13_4(v):
+    s = "%.4E" % v
+    mant, ex = s.split("E")
+    e = int(ex)
+    s = f"{mant}E{'-' if e < 0 else '+'}{abs(e):02d}" if abs(e) < 100 else f"{mant}{'-' if e < 0 else '+'}{abs(e):03d}"
+    return s.rjust(13)
+
+
+def eigen_table(prob, evals, inten, bji, bconv=0):
+    """The output.dat section written by compute_eigenvalues
+    (Source/top.f90:1057-1112), as text."""
+    keep = select_roots(evals, inten, bji, bconv)
+    zpve = sum(0.5 * w for w in prob.aomega)
+    imax = max(inten[k] for k in keep)
+    lines = ["", "COMPUTED EIGENVALUES ", "------------------------------", "",
+             "  ZPVE from nadvibs.in (reference state): %12.4f cm-1" % (zpve * AU2WAVE),
+             "  FIRST eigenvalue:                       %12.4f cm-1" % (evals[keep[0]] * AU2WAVE),
+             "  Intensity Factor (MAX Intensity):       %12.8f" % imax, "",
+             "  Root  Index  Convergence     E(cm-1)        E(eV)     Intensity"]
+    for r, k in enumerate(keep):
+        lines.append("%6d%7d%s%15.5f%12.5f%12.7f" % (r + 1, k + 1, _es13_4(bji[k]),
+                                                     (evals[k] - evals[keep[0]]) * AU2WAVE,
+                                                     (evals[k] - evals[0]) * AU2EV, inten[k] / imax))
+    lines += ["", "   All requested eigenvalues computed."]
+    return "\n".join(lines) + "\n"
+
+
+def write_restart_log(path, alpha, beta, omega=None):
+    """restart.log as make_restart writes it (Source/top.f90:1020-1039)."""
+    n = len(alpha)
+    fmt = lambda a: " " + " ".join("%.17E" % v for v in a)
+    out = [" TOTAL NUMBER OF ITERATIONS IN FILE", " %d" % n, " ITERATION NUMBER TO RESTART FROM", " %d" % n,
+           " ALPHA COEFFICIENTS -- diagonal T matrix elements", fmt(alpha),
+           " BETA COEFFICIENTS -- off-diagonal T matrix elements", fmt(beta)]
+    if omega is not None:
+        out += [" OMEGA(1) -- ORTHOGONALITY INFORMATION", fmt(omega[0]),
+                " OMEGA(2) -- ORTHOGONALITY INFORMATION", fmt(omega[1])]
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def read_restart_log(path, orthog=False):
+    """Reads restart.log (Source/secondary.f90:872-892): returns (total,
+    restart_iter, alpha(1:n), beta(0:n), omega or None)."""
+    from .inputs import _ListReader
+    with open(path) as f:
+        rd = _ListReader(f.read().splitlines())
+    rd.skip(); tot = int(rd.read(1)[0])
+    rd.skip(); n = int(rd.read(1)[0])
+    rd.skip(); a = rd.read(n)
+    rd.skip(); b = rd.read(n + 1)
+    om = None
+    if orthog:
+        rd.skip(); o1 = rd.read(n + 1)
+        rd.skip(); o2 = rd.read(n + 1)
+        om = np.vstack([o1, o2])
+    return tot, n, a, b, om

@@ -1,0 +1,142 @@
+"""ctypes wrapper of liboracle.so (TEST INFRASTRUCTURE ONLY).
+
+Mirrors the reference functions restated in nv_oracle.c; see that file for
+the citations.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class _Problem(ctypes.Structure):
+    # layout of nv_problem in include/nadvibs.h
+    _fields_ = [
+        ("nstates", ctypes.c_int64), ("nmodes", ctypes.c_int64), ("ordr", ctypes.c_int64),
+        ("nfunc", _i64p), ("aomega", _dp), ("concoef", _dp), ("potterms", _dp),
+        ("ztoler", ctypes.c_double), ("quirks", ctypes.c_int32), ("neworigin", ctypes.c_int32),
+        ("statew", _dp), ("bomega", _dp), ("dvec", _dp), ("tmat", _dp),
+        ("nirreps", ctypes.c_int64), ("npirrep", _i64p), ("istate", _i64p),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        l = ctypes.CDLL(LIB)
+        l.or_create.restype = ctypes.c_void_p
+        l.or_create.argtypes = [ctypes.POINTER(_Problem)]
+        l.or_destroy.argtypes = [ctypes.c_void_p]
+        l.or_dims.restype = ctypes.c_long
+        l.or_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        l.or_term_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        l.or_term_summary.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        l.or_hv.argtypes = [ctypes.c_void_p, _dp, _dp]
+        l.or_start_vector.argtypes = [ctypes.c_void_p, _dp]
+        l.or_lanczos.restype = ctypes.c_int
+        l.or_lanczos.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_int, ctypes.c_long, ctypes.c_uint64,
+                                 ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long)]
+        l.or_givens.argtypes = [ctypes.c_long, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int), _dp]
+        _lib = l
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+class Oracle:
+    """FP64 CPU restatement for one problem (an inputs.Problem-like object)."""
+
+    def __init__(self, prob):
+        self.prob = p = prob
+        self._keep = [np.ascontiguousarray(p.nfunc, dtype=np.int64),
+                      np.ascontiguousarray(p.aomega, dtype=np.float64),
+                      np.ascontiguousarray(p.concoef, dtype=np.float64),
+                      np.ascontiguousarray(p.potterms, dtype=np.float64),
+                      np.ascontiguousarray(p.statew if p.statew is not None else np.ones(p.nstates),
+                                           dtype=np.float64)]
+        nf, ao, cc, pt, sw = self._keep
+        desc = _Problem(nstates=p.nstates, nmodes=p.nmodes, ordr=p.ordr, nfunc=nf.ctypes.data_as(_i64p),
+                        aomega=_d(ao), concoef=_d(cc), potterms=_d(pt), ztoler=p.ztoler, quirks=p.quirks,
+                        neworigin=0, statew=_d(sw), nirreps=1)
+        self.h = lib().or_create(ctypes.byref(desc))
+        if not self.h:
+            raise ValueError("oracle rejected the problem")
+        D = ctypes.c_long()
+        self.n = lib().or_dims(self.h, ctypes.byref(D))
+        self.dimen = D.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_destroy(self.h)
+            self.h = None
+
+    def hv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        lib().or_hv(self.h, _d(x), _d(y))
+        return y
+
+    def start_vector(self):
+        q = np.empty(self.n)
+        lib().or_start_vector(self.h, _d(q))
+        return q
+
+    def term_counts(self):
+        out = np.zeros(self.prob.ordr * self.prob.nstblks, dtype=np.int64)
+        lib().or_term_counts(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        return out.reshape(self.prob.ordr, self.prob.nstblks)
+
+    def term_summary(self):
+        out = np.zeros(self.prob.ordr, dtype=np.int64)
+        lib().or_term_summary(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        return out
+
+    def lanczos(self, niter, reorthog=0, chkorthog=100, seed=12345, noise=1.0, start=None, keep_basis=False):
+        a = np.zeros(niter)
+        b = np.zeros(niter + 1)
+        om = np.zeros(2 * (niter + 1))
+        st = np.zeros(2, dtype=np.int64)
+        arch = np.zeros((niter + 1) * self.n) if keep_basis else None
+        s = None if start is None else np.ascontiguousarray(start, dtype=np.float64)
+        rc = lib().or_lanczos(self.h, niter, reorthog, chkorthog, seed, noise, _d(s), _d(a), _d(b), _d(om),
+                              _d(arch), st.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        if rc:
+            raise MemoryError("oracle Lanczos could not allocate its Krylov basis")
+        out = dict(alpha=a, beta=b, omega=om.reshape(2, niter + 1), events=int(st[0]), touched=int(st[1]))
+        if keep_basis:
+            out["basis"] = arch.reshape(niter + 1, self.n)[1:]
+        return out
+
+
+def givens(alpha, beta, vectors=False):
+    """GIVENS restated for the tridiagonal T(alpha, beta(1:n-1)); beta holds
+    beta(0:n).  Returns (eval, inten, bji, sign[, vecs])."""
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    n = alpha.size
+    ev, it, bj = np.empty(n), np.empty(n), np.empty(n)
+    sg = np.empty(n, dtype=np.int32)
+    vec = np.empty(n * n) if vectors else None
+    lib().or_givens(n, _d(alpha), _d(beta), _d(ev), _d(it), _d(bj), sg.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                    _d(vec))
+    if vectors:
+        return ev, it, bj, sg, vec.reshape(n, n)
+    return ev, it, bj, sg

@@ -1,0 +1,58 @@
+"""The C ABI library loads, exports every entry point include/nadvibs.h
+declares, and rejects invalid problems with a status code and a message
+(no GPU needed: validation happens before any device call)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from nadvibs_amd import _lib
+from nadvibs_amd.synth import make_problem
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    text = open(os.path.join(ROOT, "include", "nadvibs.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nv_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = _lib.load()
+    names = declared_entry_points()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(n for n, _, _ in _lib.SIGNATURES) == names
+
+
+def _desc(p, **over):
+    keep = [np.ascontiguousarray(p.nfunc, dtype=np.int64), np.ascontiguousarray(p.aomega),
+            np.ascontiguousarray(p.concoef), np.ascontiguousarray(p.potterms)]
+    d = dict(nstates=p.nstates, nmodes=p.nmodes, ordr=p.ordr, nfunc=_lib.iptr(keep[0]), aomega=_lib.dptr(keep[1]),
+             concoef=_lib.dptr(keep[2]), potterms=_lib.dptr(keep[3]), ztoler=1e-20, quirks=3, nirreps=1)
+    d.update(over)
+    return _lib.NvProblem(**d), keep
+
+
+@pytest.mark.parametrize("over,msg", [(dict(ordr=5), "ordr"), (dict(nstates=0), "nstates"),
+                                      (dict(nstates=11), "nstates"), (dict(nmodes=0), "nmodes")])
+def test_invalid_problem_is_rejected(over, msg):
+    lib = _lib.load()
+    p = make_problem(2, [3, 3], 2, (500, 900), 10.0, 5.0, 5.0, seed=1)
+    desc, keep = _desc(p, **over)
+    h = ctypes.c_void_p()
+    rc = lib.nv_create(ctypes.byref(desc), 0, ctypes.byref(h))
+    assert rc == 1 and not h.value
+    assert msg in lib.nv_last_error(None).decode()
+
+
+def test_null_arguments():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.nv_create(None, 0, ctypes.byref(h)) == 1
+    assert lib.nv_destroy(None) == 0
+    assert lib.nv_dims(None, None, None) == 1
