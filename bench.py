@@ -1,0 +1,175 @@
+"""Benchmark: Lanczos iterations/s and H*v bandwidth on one MI355X.
+
+Workload (default): nstates=3, nmodes=6, basis [12,10,10,8,8,6] (N=1,382,400),
+ordr=2, niter=2000, partial re-orthogonalisation (reorthog=1, chkorthog=100),
+seeded synthetic couplings.  One step = one whole Lanczos job: start vector,
+2000 iterations (H*v, recurrence, omega recurrence, re-orthogonalisation) and
+the tridiagonal eigensolve with intensities.  The mixed-precision job (H*v
+FP32, recurrence FP64) is the headline; the FP64 job is timed beside it and
+the converged eigenvalues of the two are compared (cm^-1).
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Lanczos iter/s + H*v HBM GB/s (%peak); mixed vs FP64 at <=1e-3 cm^-1 error"
+HBM_PEAK_GBS = 8000.0
+AU2WAVE = 219474.6313705
+
+
+def run_job(eng, niter, prec, reorthog, chkorthog, seed):
+    eng.lanczos_init(niter=niter, reorthog=reorthog, prec=prec, chkorthog=chkorthog, seed=seed)
+    eng.run(1, niter)
+    a, b, _ = eng.coefficients()
+    ev, it, bj, _ = eng.ritz(a, b)
+    return a, b, ev, it, bj
+
+
+def timed(eng, args, prec, sync):
+    for k in range(args.warmup):
+        t0 = time.perf_counter()
+        run_job(eng, args.niter, prec, args.reorthog, args.chkorthog, args.seed)
+        print(f"# {prec} warmup {k + 1}/{args.warmup}: {time.perf_counter() - t0:.2f} s", file=sys.stderr,
+              flush=True)
+    eng.profile(True)
+    eng.profile_read()
+    sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = run_job(eng, args.niter, prec, args.reorthog, args.chkorthog, args.seed)
+        print(f"# {prec} step {k + 1}/{args.steps} done at {time.perf_counter() - t0:.2f} s", file=sys.stderr,
+              flush=True)
+    sync()
+    dt = time.perf_counter() - t0
+    hv_ms, hv_n = eng.profile_read()
+    eng.profile(False)
+    return dt, hv_ms, hv_n, out, eng.pro_stats()
+
+
+def compare(ev64, it64, bj64, ev, it, bj, tol_conv=1e-9):
+    """Converged FP64 roots matched to the nearest root of the other run."""
+    conv = np.nonzero(bj64 < tol_conv)[0]
+    dE, dI = [], []
+    for k in conv:
+        j = int(np.argmin(np.abs(ev - ev64[k])))
+        dE.append(abs(ev[j] - ev64[k]) * AU2WAVE)
+        dI.append(abs(it[j] - it64[k]))
+    return (max(dE) if dE else float("nan")), (max(dI) if dI else float("nan")), len(conv)
+
+
+def cpu_baseline(prob, seconds):
+    """The oracle (scalar FP64 restatement) on this host's CPU, 1 thread, on
+    the first iterations of the same job."""
+    from oracle.oracle import Oracle
+    orc = Oracle(prob)
+    t0 = time.perf_counter()
+    orc.hv(np.ones(orc.n))
+    t_hv = time.perf_counter() - t0
+    k = max(2, int(seconds / max(t_hv * 1.1, 1e-6)))
+    t0 = time.perf_counter()
+    orc.lanczos(k, reorthog=prob.reorthog, chkorthog=prob.chkorthog)
+    dt = time.perf_counter() - t0
+    return dict(value=k / dt, unit="Lanczos iter/s", cores=1, kind="port",
+                sample=f"first {k} Lanczos iterations of the same workload (FP64 oracle, 1 thread), {dt:.1f} s")
+
+
+def load_traffic(path):
+    """HBM bytes per H*v launch measured with rocprofv3 counters, if a
+    summary with a 'traffic_bytes_per_launch:' line exists."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--niter", type=int, default=None)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the engine runs on a single "
+              f"MI355X; multi-GPU sharding is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from nadvibs_amd.engine import Engine
+    from nadvibs_amd.synth import workload
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    prob = workload(args.workload)
+    args.niter = args.niter or prob.niter
+    args.reorthog, args.chkorthog, args.seed = prob.reorthog, prob.chkorthog, 2024
+    sync = torch.cuda.synchronize
+    N = prob.n
+    print(f"# workload {args.workload}: N={N} niter={args.niter} reorthog={args.reorthog}", file=sys.stderr,
+          flush=True)
+    with Engine(prob, device=0) as eng:
+        dt32, hv32_ms, hv32_n, out32, pro32 = timed(eng, args, "f32", sync)
+        print(f"# f32 job {dt32 / args.steps * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        res64 = None
+        if not args.no_fp64:
+            dt64, hv64_ms, hv64_n, out64, pro64 = timed(eng, args, "f64", sync)
+            res64 = (dt64, hv64_ms, hv64_n, out64, pro64)
+            print(f"# f64 job {dt64 / args.steps * 1e3:.1f} ms", file=sys.stderr, flush=True)
+
+    iters = args.steps * args.niter
+    value = iters / dt32
+    hv_avg_s = hv32_ms / max(hv32_n, 1) / 1e3
+    hv_bytes = N * (4 + 4)  # x in FP32 read, off-diagonal y in FP32 written
+    achieved = hv_bytes / hv_avg_s / 1e9
+    rec = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Lanczos iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt32 / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: nstates={prob.nstates} nmodes={prob.nmodes} "
+                               f"basis={prob.nfunc} ordr={prob.ordr} N={N}",
+                   "niter": args.niter, "reorthog": args.reorthog, "chkorthog": args.chkorthog,
+                   "hv_precision": "f32", "recurrence_precision": "f64", "step": "one whole Lanczos job"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(os.path.join(ROOT, "profiles", f"hv_f32_{args.workload}_traffic.txt")),
+                     "kernel": "hv_iter_kernel<3,float,float>", "avg_launch_us": round(hv_avg_s * 1e6, 2),
+                     "bytes_per_launch": hv_bytes, "launches": hv32_n},
+        "hv_gbps": round(achieved, 1),
+        "pro": {"events_per_job": pro32[0], "vectors_touched_per_job": pro32[1]},
+    }
+    if res64 is not None:
+        dt64, hv64_ms, hv64_n, out64, _ = res64
+        v64 = iters / dt64
+        hv64_s = hv64_ms / max(hv64_n, 1) / 1e3
+        dE, dI, nconv = compare(out64[2], out64[3], out64[4], out32[2], out32[3], out32[4])
+        rec["fp64"] = {"value": round(v64, 2), "ms_per_step": round(dt64 / args.steps * 1e3, 3),
+                       "hv_avg_launch_us": round(hv64_s * 1e6, 2),
+                       "hv_gbps": round(N * 16 / hv64_s / 1e9, 1)}
+        rec["speedup_vs_fp64"] = round(value / v64, 3)
+        rec["max_abs_dE_cm1"] = float(f"{dE:.3e}")
+        rec["max_abs_dI"] = float(f"{dI:.3e}")
+        rec["n_converged_fp64"] = nconv
+        rec["error_target_met"] = bool(dE <= 1e-3)
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

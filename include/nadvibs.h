@@ -121,9 +121,15 @@ typedef struct nv_lanczos_opts {
     int32_t reorthog;         /* basis.in reorthog: 0 none, 1 partial with the
                                  omega recurrence, >1 partial with exact overlaps */
     int32_t prec;             /* NV_PREC_* for H*v */
-    int64_t chkorthog;        /* exact overlap check period (default 100) */
+    int64_t chkorthog;        /* exact overlap check period (default 100); with
+                                 NV_PREC_F32/BF16 the period is min(chkorthog, 10):
+                                 the omega model assumes FP64 rounding of H*v and
+                                 the checks keep it honest */
     uint64_t seed;            /* seed of the omega-estimate noise */
     double noise_scale;       /* multiplies every noise draw (1 = reference, 0 = off) */
+    double pro_eps;           /* unit roundoff assumed by the omega recurrence and
+                                 its thresholds sqrt(eps), eps^(3/4); 0 = 2^-53,
+                                 the reference's determine_epsilon */
 } nv_lanczos_opts;
 
 /* Allocates the Krylov basis for opts->niter iterations, builds the start
@@ -174,6 +180,15 @@ int32_t nv_save_vectors(nv_ctx *ctx, const char *path);
 int32_t nv_load_vectors(nv_ctx *ctx, const char *path, int64_t iter,
                         const double *alpha, const double *beta,
                         const double *omega);
+
+/* ---- measurement hooks (no counterpart in the reference: they serve the
+ *      benchmark's live roofline) ------------------------------------------ */
+/* on != 0: every H*v launch of the Lanczos loop is bracketed by HIP events on
+ * the context's stream. */
+int32_t nv_profile_enable(nv_ctx *ctx, int32_t on);
+/* Milliseconds spent in the timed H*v launches and their number since the
+ * previous call; synchronises. */
+int32_t nv_profile_read(nv_ctx *ctx, double *hv_ms, int64_t *hv_launches);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,7 @@ class NvLanczosOpts(ctypes.Structure):
     _fields_ = [
         ("niter", ctypes.c_int64), ("reorthog", ctypes.c_int32), ("prec", ctypes.c_int32),
         ("chkorthog", ctypes.c_int64), ("seed", ctypes.c_uint64), ("noise_scale", ctypes.c_double),
+        ("pro_eps", ctypes.c_double),
     ]
 
 
@@ -68,6 +69,8 @@ SIGNATURES = [
     ("nv_ritz", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _i32p]),
     ("nv_save_vectors", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p]),
     ("nv_load_vectors", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64, _dp, _dp, _dp]),
+    ("nv_profile_enable", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    ("nv_profile_read", ctypes.c_int32, [ctypes.c_void_p, _dp, _i64p]),
 ]
 
 _lib = None

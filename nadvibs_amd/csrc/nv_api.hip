@@ -37,6 +37,12 @@ struct nv_ctx {
 
     // graph of one iteration
     hipGraphExec_t graph = nullptr;
+
+    // H*v timing
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
+    size_t lz_bytes_key = 0;
 };
 
 static std::string g_create_error;
@@ -146,6 +152,8 @@ int32_t nv_destroy(nv_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     release_lanczos(c);
     free_list(c->allocs);
+    for (auto &p : c->ev) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+    if (c->d_startvals) hipFree(c->d_startvals);
     if (c->hx) hipFree(c->hx);
     if (c->hy) hipFree(c->hy);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -253,37 +261,45 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     if (!c || !o) return NV_ERR_ARG;
     if (o->niter < 1) return fail(c, NV_ERR_ARG, "niter must be >= 1");
     if (!valid_prec(o->prec)) return fail(c, NV_ERR_ARG, "unknown precision");
+    if (o->reorthog > 0 && nv::pro_decide_lds(o->niter) > 64 * 1024)
+        return fail(c, NV_ERR_ARG, "re-orthogonalisation supports up to 13000 iterations");
     NV_HIP(c, hipSetDevice(c->device));
     NV_HIP(c, hipStreamSynchronize(c->stream));
-    release_lanczos(c);
-    c->opts = *o;
-    if (c->opts.chkorthog < 1) c->opts.chkorthog = 100;
     const int64_t N = c->op.N;
     const bool keep = o->reorthog > 0;
+    // buffers of an earlier run with the same shape are reused
+    const bool same = c->lz_ready && c->opts.niter == o->niter && c->opts.prec == o->prec &&
+                      (c->opts.reorthog > 0) == keep && (c->opts.reorthog > 1) == (o->reorthog > 1);
     nv::LzState &st = c->st;
-    st.nslots = keep ? o->niter + 2 : 3;
-    auto alloc = [&](size_t bytes, void **p) -> hipError_t {
-        hipError_t e = hipMalloc(p, bytes);
-        if (e == hipSuccess) c->lz_allocs.push_back(*p);
-        return e;
-    };
-    NV_HIP(c, alloc((size_t)st.nslots * N * sizeof(double), (void **)&st.V));
-    const size_t el = o->prec == NV_PREC_F64 ? 8 : (o->prec == NV_PREC_F32 ? 4 : 2);
-    if (o->prec != NV_PREC_F64) NV_HIP(c, alloc((size_t)N * el, &st.vlo));
-    NV_HIP(c, alloc((size_t)N * (o->prec == NV_PREC_F64 ? 8 : 4), &st.y));
-    NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.alpha));
-    NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.beta));
-    const int64_t nparts = std::max<int64_t>(nv::hv_blocks(c->dev), 8192) + 64;
-    NV_HIP(c, alloc(nparts * sizeof(double), (void **)&st.partials));
-    NV_HIP(c, alloc(nv::kNumCounters * sizeof(unsigned), (void **)&st.counters));
-    NV_HIP(c, alloc(sizeof(int64_t), (void **)&c->d_iter));
-    st.iter = c->d_iter;
+    if (!same) {
+        release_lanczos(c);
+        st.nslots = keep ? o->niter + 2 : 3;
+        auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+            hipError_t e = hipMalloc(p, bytes);
+            if (e == hipSuccess) c->lz_allocs.push_back(*p);
+            return e;
+        };
+        NV_HIP(c, alloc((size_t)st.nslots * N * sizeof(double), (void **)&st.V));
+        const size_t el = o->prec == NV_PREC_F64 ? 8 : (o->prec == NV_PREC_F32 ? 4 : 2);
+        if (o->prec != NV_PREC_F64) NV_HIP(c, alloc((size_t)N * el, &st.vlo));
+        NV_HIP(c, alloc((size_t)N * (o->prec == NV_PREC_F64 ? 8 : 4), &st.y));
+        NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.alpha));
+        NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.beta));
+        const int64_t nparts = std::max<int64_t>(nv::hv_blocks(c->dev), 8192) + 64;
+        NV_HIP(c, alloc(nparts * sizeof(double), (void **)&st.partials));
+        NV_HIP(c, alloc(nv::kNumCounters * sizeof(unsigned), (void **)&st.counters));
+        NV_HIP(c, alloc(sizeof(int64_t), (void **)&c->d_iter));
+        st.iter = c->d_iter;
+    }
+    c->opts = *o;
+    if (c->opts.chkorthog < 1) c->opts.chkorthog = 100;
     NV_HIP(c, hipMemsetAsync(st.counters, 0, nv::kNumCounters * sizeof(unsigned), c->stream));
     NV_HIP(c, hipMemsetAsync(st.alpha, 0, (o->niter + 3) * sizeof(double), c->stream));
     NV_HIP(c, hipMemsetAsync(st.beta, 0, (o->niter + 3) * sizeof(double), c->stream));
     NV_HIP(c, hipMemsetAsync(st.V, 0, (size_t)std::min<int64_t>(st.nslots, 3) * N * sizeof(double), c->stream));
     if (keep) {
-        hipError_t e = nv::pro_init(c->pro, c->opts, c->op, c->dev, st, c->lz_allocs, c->stream);
+        hipError_t e = same ? nv::pro_reset(c->pro, c->opts, c->stream)
+                            : nv::pro_init(c->pro, c->opts, c->op, c->dev, st, c->lz_allocs, c->stream);
         if (e != hipSuccess) return hip_fail(c, e, "pro_init");
     }
     // start vector in the slot of iteration 1, beta(0) = |start|
@@ -319,7 +335,42 @@ int32_t nv_hv_iter(nv_ctx *c, int64_t it) {
     int32_t rc = check_iter(c, it);
     if (rc) return rc;
     if (it != c->next_update - 1) return fail(c, NV_ERR_STATE, "Hv called before update_lanczos");
+    if (c->prof) {
+        if (c->ev_used == c->ev.size()) {
+            hipEvent_t a, b;
+            NV_HIP(c, hipEventCreate(&a));
+            NV_HIP(c, hipEventCreate(&b));
+            c->ev.emplace_back(a, b);
+        }
+        NV_HIP(c, hipEventRecord(c->ev[c->ev_used].first, c->stream));
+    }
     NV_HIP(c, nv::launch_hv_iter(c->dev, c->opts.prec, c->st, c->stream));
+    if (c->prof) {
+        NV_HIP(c, hipEventRecord(c->ev[c->ev_used].second, c->stream));
+        c->ev_used++;
+    }
+    return NV_OK;
+}
+
+int32_t nv_profile_enable(nv_ctx *c, int32_t on) {
+    if (!c) return NV_ERR_ARG;
+    c->prof = on != 0;
+    return NV_OK;
+}
+
+int32_t nv_profile_read(nv_ctx *c, double *ms, int64_t *count) {
+    if (!c) return NV_ERR_ARG;
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    double t = 0.0;
+    for (size_t i = 0; i < c->ev_used; ++i) {
+        float f = 0.0f;
+        NV_HIP(c, hipEventElapsedTime(&f, c->ev[i].first, c->ev[i].second));
+        t += f;
+    }
+    if (ms) *ms = t;
+    if (count) *count = (int64_t)c->ev_used;
+    c->ev_used = 0;
     return NV_OK;
 }
 
@@ -412,6 +463,9 @@ int32_t nv_ritz(nv_ctx *c, int64_t n, const double *alpha, const double *beta, d
                 double *bji, int32_t *sign) {
     if (!c) return NV_ERR_ARG;
     if (n < 1 || !alpha || !beta || !eval) return fail(c, NV_ERR_ARG, "bad tridiagonal arguments");
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(alpha[i]) || !std::isfinite(beta[i + 1]))
+            return fail(c, NV_ERR_ARG, "alpha/beta hold a non-finite value at iteration " + std::to_string(i + 1));
     NV_HIP(c, hipSetDevice(c->device));
     std::vector<double> root(n), v1(n), vn(n);
     // Frobenius scaling in the order GIVENS accumulates it (Source/yarkony.f90:221-247)

@@ -488,7 +488,9 @@ __device__ double tri_backsolve(int64_t n, int64_t i, const double *U2, const do
             const double v1 = l + 1 < n ? v[(l + 1) * n + i] : 0.0;
             const double v2 = l + 2 < n ? v[(l + 2) * n + i] : 0.0;
             const double e = v[l * n + i] - v1 * U3[l * n + i] - v2 * U4[l * n + i];
-            if (fabs(e) > NV_DELBIG) {
+            // overflow guard of the reference; a non-finite value cannot be
+            // rescaled and ends the sweep instead of looping
+            if (fabs(e) > NV_DELBIG && isfinite(e)) {
                 for (int64_t k = 0; k < n; ++k) v[k * n + i] /= NV_DELBIG;
                 continue;
             }
@@ -578,37 +580,50 @@ __global__ void tri_inverse_kernel(int64_t n, const double *d, const double *e, 
     tri_backsolve(n, i, U2, U3, U4, vec);
 }
 
-__device__ void tri_normalize(int64_t n, int64_t i, double *vec) {
+// Normalisation (Source/yarkony.f90:900-920) by one block: max |v|, sum of
+// (v/max)^2, scale, flush components below eps/100 to zero.
+__device__ void tri_normalize_block(int64_t n, int64_t i, double *vec) {
+    __shared__ double red[16];
     double mx = 0.0;
-    for (int64_t j = 0; j < n; ++j) mx = fmax(fabs(vec[j * n + i]), mx);
+    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) mx = fmax(fabs(vec[j * n + i]), mx);
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mx = fmax(mx, red[w]);
     double s = 0.0;
-    for (int64_t j = 0; j < n; ++j) {
+    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
         const double q = vec[j * n + i] / mx;
         s += q * q;
     }
+    s = block_sum(s);
     const double t = 1.0 / (sqrt(s) * mx);
-    for (int64_t j = 0; j < n; ++j) {
+    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
         double v = vec[j * n + i] * t;
         if (fabs(v) < NV_DEL1) v = 0.0;
         vec[j * n + i] = v;
     }
+    __syncthreads();
 }
 
 // Gram-Schmidt inside clusters of near-equal roots (Source/yarkony.f90:800-870)
-// and normalisation (:900-920).  A cluster is handled by the thread of its
-// first root, member by member.
-__global__ void tri_finish_kernel(int64_t n, const int32_t *ia, double *vec) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// and normalisation.  One block per root that starts a cluster (or stands
+// alone); the block walks the cluster's members in order.
+__global__ __launch_bounds__(256) void tri_finish_kernel(int64_t n, const int32_t *ia, double *vec) {
+    const int64_t i = blockIdx.x;
     if (i >= n || ia[i] != 0) return;
-    tri_normalize(n, i, vec);
+    tri_normalize_block(n, i, vec);
     for (int64_t m = i + 1; m < n && ia[m] > 0; ++m) {
         for (int64_t j1 = 1; j1 <= ia[m]; ++j1) {
             const int64_t k = m - j1;
             double t = 0.0;
-            for (int64_t j = 0; j < n; ++j) t += vec[j * n + m] * vec[j * n + k];
-            for (int64_t j = 0; j < n; ++j) vec[j * n + m] -= t * vec[j * n + k];
+            for (int64_t j = threadIdx.x; j < n; j += blockDim.x) t += vec[j * n + m] * vec[j * n + k];
+            t = block_sum(t);
+            for (int64_t j = threadIdx.x; j < n; j += blockDim.x) vec[j * n + m] -= t * vec[j * n + k];
+            __syncthreads();
         }
-        tri_normalize(n, m, vec);
+        tri_normalize_block(n, m, vec);
     }
 }
 
@@ -632,7 +647,7 @@ hipError_t launch_tridiag(int64_t n, const TriWork &w, double anorm, hipStream_t
     hipLaunchKernelGGL(tri_bisect_kernel, dim3(g), dim3(tb), 0, s, n, w.d, w.e2, w.root);
     hipLaunchKernelGGL(tri_cluster_kernel, dim3(1), dim3(64), 0, s, n, w.root, w.ia, w.rank);
     hipLaunchKernelGGL(tri_inverse_kernel, dim3(g), dim3(tb), 0, s, n, w.d, w.e, w.root, w.ia, w.rank, w.lu, w.vec);
-    hipLaunchKernelGGL(tri_finish_kernel, dim3(g), dim3(tb), 0, s, n, w.ia, w.vec);
+    hipLaunchKernelGGL(tri_finish_kernel, dim3((unsigned)n), dim3(256), 0, s, n, w.ia, w.vec);
     return hipGetLastError();
 }
 

@@ -13,7 +13,7 @@
 
 namespace nv {
 
-#define PRO_EPS 1.1102230246251565e-16  /* 2^-53 (determine_epsilon, Source/secondary.f90:260-271) */
+#define PRO_EPS64 1.1102230246251565e-16  /* 2^-53 (determine_epsilon, Source/secondary.f90:260-271) */
 
 __device__ __forceinline__ double lz_beta(const LzState &st, int64_t i) { return st.beta[i]; }
 
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void pro_recur_kernel(ProDev p, LzState st)
     for (int64_t k = 1 + threadIdx.x; k <= it - 1; k += blockDim.x) {
         o1[k] = (1 / bi) * (st.beta[k] * o2[k + 1] + (st.alpha[k] - st.alpha[it]) * o2[k] +
                             st.beta[k - 1] * o2[k - 1] - st.beta[it - 1] * o1[k]) +
-                PRO_EPS * (st.beta[k] + bi) * (p.noise * noise_gauss(p.seed, base + (uint64_t)k, 0.3));
+                p.eps * (st.beta[k] + bi) * (p.noise * noise_gauss(p.seed, base + (uint64_t)k, 0.3));
     }
     __syncthreads();
     for (int64_t i = 1 + threadIdx.x; i <= it; i += blockDim.x) {
@@ -61,53 +61,104 @@ __global__ __launch_bounds__(kBlock) void pro_recur_kernel(ProDev p, LzState st)
 }
 
 // Dots of a and b (either may be null) with the Krylov vectors q_j listed in
-// jlist (or j = c+1 when jlist is null), c < count.  Each wave writes one
-// partial per dot; pro_reduce_kernel sums them in a fixed order.
+// jlist (or j = c+1 when jlist is null), c < count: a block GEMV over the
+// HBM-resident basis.  Work items are (group of kGemvG vectors) x (chunk of
+// kGemvCh elements); a persistent grid walks them, so the launch shape does
+// not depend on the device-side count.  Each item writes one partial per dot;
+// pro_reduce_kernel sums the chunks of a dot in chunk order.
+constexpr int kGemvG = 8;
+constexpr int kGemvE = 8;
+constexpr int kGemvCh = kBlock * kGemvE;
+
 __global__ __launch_bounds__(kBlock) void pro_multidot_kernel(int64_t N, LzState st, const int64_t *jlist,
                                                               const int32_t *count_dev, int32_t count_host,
-                                                              const double *a_fixed, int a_cur, int b_res,
-                                                              double *part, int32_t nchunks) {
+                                                              int a_cur, int b_res, double *part) {
+    __shared__ double red[kBlock / 64][2 * kGemvG];
     const int64_t it = *st.iter;
     const int32_t count = count_dev ? *count_dev : count_host;
-    const double *a = a_fixed ? a_fixed : (a_cur ? st.V + (it % st.nslots) * N : nullptr);
+    if (count <= 0) return;
+    const double *a = a_cur ? st.V + (it % st.nslots) * N : nullptr;
     const double *b = b_res ? st.V + ((it + 1) % st.nslots) * N : nullptr;
-    const int64_t chunk = (N + nchunks - 1) / nchunks;
-    const int64_t lo = (int64_t)blockIdx.x * chunk;
-    const int64_t hi = lo + chunk < N ? lo + chunk : N;
+    const int64_t nchunks = (N + kGemvCh - 1) / kGemvCh;
+    const int64_t ngroups = (count + kGemvG - 1) / kGemvG;
+    const int64_t items = ngroups * nchunks;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int64_t g = (int64_t)blockIdx.x * (kBlock / 64) + wid;
-    const int64_t nw = (int64_t)nchunks * (kBlock / 64);
-    for (int32_t c = 0; c < count; ++c) {
-        const int64_t j = jlist ? jlist[c] : (int64_t)c + 1;
-        const double *q = st.V + (j % st.nslots) * N;
+    for (int64_t item = blockIdx.x; item < items; item += gridDim.x) {
+        const int64_t g = item / nchunks, ch = item - g * nchunks;
+        const int64_t base = ch * kGemvCh + threadIdx.x;
+        double av[kGemvE], bv[kGemvE];
+#pragma unroll
+        for (int e = 0; e < kGemvE; ++e) {
+            const int64_t i = base + (int64_t)e * kBlock;
+            av[e] = (a && i < N) ? a[i] : 0.0;
+            bv[e] = (b && i < N) ? b[i] : 0.0;
+        }
+        double sa[kGemvG], sb[kGemvG];
+#pragma unroll
+        for (int k = 0; k < kGemvG; ++k) {
+            sa[k] = 0.0;
+            sb[k] = 0.0;
+            const int64_t c = g * kGemvG + k;
+            if (c < count) {
+                const int64_t j = jlist ? jlist[c] : c + 1;
+                const double *q = st.V + (j % st.nslots) * N;
+#pragma unroll
+                for (int e = 0; e < kGemvE; ++e) {
+                    const int64_t i = base + (int64_t)e * kBlock;
+                    const double qi = i < N ? q[i] : 0.0;
+                    sa[k] += av[e] * qi;
+                    sb[k] += bv[e] * qi;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kGemvG; ++k) {
+            for (int off = 32; off > 0; off >>= 1) {
+                sa[k] += __shfl_xor(sa[k], off, 64);
+                sb[k] += __shfl_xor(sb[k], off, 64);
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < kGemvG; ++k) {
+                red[wid][k] = sa[k];
+                red[wid][kGemvG + k] = sb[k];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * kGemvG) {
+            const int k = threadIdx.x % kGemvG, which = threadIdx.x / kGemvG;
+            const int64_t c = g * kGemvG + k;
+            double t = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
+            if (c < count) part[(c * nchunks + ch) * 2 + which] = t;
+        }
+    }
+}
+
+// One wave per dot: sums its chunk partials in chunk order.
+__global__ __launch_bounds__(kBlock) void pro_reduce_kernel(int64_t N, const int32_t *count_dev, int32_t count_host,
+                                                            const double *part, double *out_a, double *out_b) {
+    const int32_t count = count_dev ? *count_dev : count_host;
+    const int64_t nchunks = (N + kGemvCh - 1) / kGemvCh;
+    const int lane = threadIdx.x & 63;
+    const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t c = w0; c < count; c += nw) {
         double sa = 0.0, sb = 0.0;
-        for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-            const double qi = q[i];
-            if (a) sa += a[i] * qi;
-            if (b) sb += b[i] * qi;
+        for (int64_t k = lane; k < nchunks; k += 64) {
+            sa += part[(c * nchunks + k) * 2 + 0];
+            sb += part[(c * nchunks + k) * 2 + 1];
         }
         for (int off = 32; off > 0; off >>= 1) {
             sa += __shfl_xor(sa, off, 64);
             sb += __shfl_xor(sb, off, 64);
         }
         if (lane == 0) {
-            part[((int64_t)c * nw + g) * 2 + 0] = sa;
-            part[((int64_t)c * nw + g) * 2 + 1] = sb;
+            out_a[c] = sa;
+            out_b[c] = sb;
         }
-    }
-}
-
-__global__ void pro_reduce_kernel(const int32_t *count_dev, int32_t count_host, const double *part, int64_t nw,
-                                  double *out_a, double *out_b) {
-    const int32_t count = count_dev ? *count_dev : count_host;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < count; c += (int64_t)gridDim.x * blockDim.x) {
-        double sa = 0.0, sb = 0.0;
-        for (int64_t g = 0; g < nw; ++g) {
-            sa += part[(c * nw + g) * 2 + 0];
-            sb += part[(c * nw + g) * 2 + 1];
-        }
-        out_a[c] = sa;
-        out_b[c] = sb;
     }
 }
 
@@ -134,68 +185,104 @@ __global__ void pro_exact_kernel(ProDev p, LzState st) {
         o2[i] = i == 0 ? 0.0 : p.dots_b[i - 1];
 }
 
-// Range detection of partial_orthog (Source/top.f90:924-970), one thread.
-__global__ void pro_decide_kernel(ProDev p, LzState st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Range detection of partial_orthog (Source/top.f90:924-970).  The reference
+// walks left and right from every index with |omega| >= sqrt(eps) to the
+// nearest |omega| < eps^(3/4); the same edges come from one backward scan
+// (next small index) and the running last small index, so the single-thread
+// part is O(iter) over LDS.  The list of vectors and the omega resets are
+// then written by the whole block, range after range so that an index in two
+// ranges keeps the draw of the later one, as in the reference.
+__global__ __launch_bounds__(kBlock) void pro_decide_kernel(ProDev p, LzState st) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int64_t sh_n, sh_base, sh_nr;
     const int64_t it = *st.iter;
     double *o2 = p.omega + p.ostride;
-    const double seps = sqrt(PRO_EPS);
-    const double eta = pow(PRO_EPS, 0.75);
-    int64_t firststep = p.meta[0], northog = p.meta[1];
-    if (firststep) {
-        northog = 0;
-        int64_t prev_lo = 0, prev_ro = 0;
-        for (int64_t i = 1; i <= it; ++i) {
-            if (!(fabs(o2[i]) >= seps)) continue;
-            int64_t l = i;
-            while (l >= 1 && !(fabs(o2[l]) < eta)) --l;
-            if (l < 1) l = 1;
-            int64_t r = i;
-            while (r <= it && !(fabs(o2[r]) < eta)) ++r;
-            if (r > it) r = it;
-            if (northog == 0 || (l != prev_lo && r != prev_ro)) {
-                if (northog < p.cap) {
-                    p.lo[northog] = l;
-                    p.ro[northog] = r;
-                    ++northog;
-                } else {
-                    p.meta[5] = 1;
+    uint8_t *cls = lds;                                            // [it+2]
+    int32_t *nxt = reinterpret_cast<int32_t *>(lds + ((it + 2 + 15) & ~int64_t(15)));  // [it+2]
+    const double seps = p.seps, eta = p.eta;
+    for (int64_t i = threadIdx.x; i <= it; i += blockDim.x) {
+        const double w = fabs(o2[i]);
+        cls[i] = (uint8_t)((w < eta ? 1 : 0) | (w >= seps ? 2 : 0));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t firststep = p.meta[0], northog = p.meta[1];
+        if (firststep) {
+            int32_t nx = (int32_t)(it + 1);
+            for (int64_t i = it; i >= 1; --i) {
+                if (cls[i] & 1) nx = (int32_t)i;
+                nxt[i] = nx;
+            }
+            northog = 0;
+            int64_t prev_lo = 0, prev_ro = 0, last_small = 0;
+            for (int64_t i = 1; i <= it; ++i) {
+                if (cls[i] & 1) last_small = i;
+                if (!(cls[i] & 2)) continue;
+                const int64_t l = last_small >= 1 ? last_small : 1;
+                const int64_t r = nxt[i] <= it ? nxt[i] : it;
+                if (northog == 0 || (l != prev_lo && r != prev_ro)) {
+                    if (northog < p.cap) {
+                        p.lo[northog] = l;
+                        p.ro[northog] = r;
+                        ++northog;
+                    } else {
+                        p.meta[5] = 1;
+                    }
+                    prev_lo = l;
+                    prev_ro = r;
                 }
-                prev_lo = l;
-                prev_ro = r;
+            }
+            if (northog > 0) {
+                // minval(loindex, mask=loindex>0) over the stored ranges,
+                // including those left from earlier detections, as the
+                // reference's array keeps them
+                const int64_t span = northog > p.extent ? northog : p.extent;
+                int64_t mn = INT64_MAX;
+                for (int64_t k = 0; k < span && k < p.cap; ++k)
+                    if (p.lo[k] > 0 && p.lo[k] < mn) mn = p.lo[k];
+                if (mn > 1 && northog < p.cap) {
+                    p.lo[northog] = 1;
+                    p.ro[northog] = 1;
+                    ++northog;
+                }
             }
         }
+        int64_t n = 0;
+        for (int64_t k = 0; k < northog; ++k) {
+            p.roff[k] = n;
+            n += p.ro[k] - p.lo[k] + 1;
+        }
+        if (n > p.jcap) {
+            p.meta[5] = 1;
+            n = p.jcap;
+        }
+        sh_n = n;
+        sh_nr = northog;
+        sh_base = p.meta[2];
         if (northog > 0) {
-            // minval(loindex, mask=loindex>0) over the stored ranges, including
-            // those left from earlier detections, as the reference's array keeps them
-            const int64_t span = northog > p.extent ? northog : p.extent;
-            int64_t mn = INT64_MAX;
-            for (int64_t k = 0; k < span && k < p.cap; ++k)
-                if (p.lo[k] > 0 && p.lo[k] < mn) mn = p.lo[k];
-            if (mn > 1 && northog < p.cap) {
-                p.lo[northog] = 1;
-                p.ro[northog] = 1;
-                ++northog;
-            }
+            p.meta[0] = !firststep;
+            p.meta[2] = sh_base + n;
+            p.meta[3] += 1;
+            p.meta[4] += n;
+        } else {
+            p.meta[0] = firststep;
         }
+        p.meta[1] = northog;
+        *p.njl = (int32_t)n;
     }
-    int32_t n = 0;
-    if (northog > 0) {
-        for (int64_t k = 0; k < northog; ++k)
-            for (int64_t j = p.lo[k]; j <= p.ro[k]; ++j) {
-                if (n < p.jcap) p.jlist[n++] = j;
-                else p.meta[5] = 1;
-            }
-        const uint64_t base = (uint64_t)p.meta[2];
-        for (int32_t c = 0; c < n; ++c) o2[p.jlist[c]] = PRO_EPS * (p.noise * noise_gauss(p.seed, base + c, 2.0));
-        p.meta[2] = (int64_t)base + n;
-        firststep = !firststep;
-        p.meta[3] += 1;
-        p.meta[4] += n;
+    __syncthreads();
+    const int64_t n = sh_n, nr = sh_nr;
+    const uint64_t base = (uint64_t)sh_base;
+    for (int64_t k = 0; k < nr; ++k) {
+        const int64_t off = p.roff[k], lo = p.lo[k], len = p.ro[k] - p.lo[k] + 1;
+        for (int64_t t = threadIdx.x; t < len; t += blockDim.x) {
+            const int64_t c = off + t;
+            if (c >= n) break;
+            p.jlist[c] = lo + t;
+            o2[lo + t] = p.eps * (p.noise * noise_gauss(p.seed, base + (uint64_t)c, 2.0));
+        }
+        __syncthreads();
     }
-    *p.njl = n;
-    p.meta[0] = firststep;
-    p.meta[1] = northog;
 }
 
 __device__ double pro_block_sum(double v) {
@@ -210,16 +297,28 @@ __device__ double pro_block_sum(double v) {
 }
 
 // Classical Gram-Schmidt update (Source/top.f90:980-990): the residual loses
-// its components along the listed vectors, then beta(i) = |residual|.
+// its components along the listed vectors, subtracted in list order, then
+// beta(i) = |residual|.  Eight basis vectors are loaded before they are used
+// so that the loads overlap.
 __global__ __launch_bounds__(kBlock) void pro_axpy_kernel(int64_t N, ProDev p, LzState st) {
     const int32_t n = *p.njl;
     if (n == 0) return;
     const int64_t it = *st.iter;
     double *w = st.V + ((it + 1) % st.nslots) * N;
+    const double *V = st.V;
+    const int64_t ns = st.nslots;
     double part = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock) {
         double v = w[i];
-        for (int32_t c = 0; c < n; ++c) v = v - p.dots_b[c] * st.V[(p.jlist[c] % st.nslots) * N + i];
+        int32_t c = 0;
+        for (; c + 8 <= n; c += 8) {
+            double q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = V[(p.jlist[c + k] % ns) * N + i];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v = v - p.dots_b[c + k] * q[k];
+        }
+        for (; c < n; ++c) v = v - p.dots_b[c] * V[(p.jlist[c] % ns) * N + i];
         w[i] = v;
         part += v * v;
     }
@@ -244,21 +343,34 @@ __global__ __launch_bounds__(kBlock) void pro_axpy_kernel(int64_t N, ProDev p, L
     }
 }
 
+// Exact-check period: the reference's chkorthog; in the mixed modes at most
+// 10, since the omega recurrence models FP64 rounding of H*v and the FP32
+// couplings lose orthogonality faster between checks.
+static int64_t chk_period(const nv_lanczos_opts &o) {
+    int64_t c = o.chkorthog < 1 ? 100 : o.chkorthog;
+    if (o.prec != NV_PREC_F64 && c > 10) c = 10;
+    return c;
+}
+
 hipError_t pro_init(ProState &ps, const nv_lanczos_opts &o, const Operator &op, const NvOpDev &dev,
                     const LzState &st, std::vector<void *> &allocs, hipStream_t s) {
     (void)dev; (void)st;
     ProDev &p = ps.d;
     p.niter = o.niter;
-    p.chkorthog = o.chkorthog < 1 ? 100 : o.chkorthog;
+    p.chkorthog = chk_period(o);
     p.exact = o.reorthog > 1;
     p.seed = o.seed;
     p.noise = o.noise_scale;
-    p.epsn = PRO_EPS * (double)op.ns * (double)op.D;
+    p.eps = o.pro_eps > 0 ? o.pro_eps : PRO_EPS64;
+    p.seps = sqrt(p.eps);
+    p.eta = pow(p.eps, 0.75);
+    p.epsn_scale = (double)op.ns * (double)op.D;
+    p.epsn = p.eps * p.epsn_scale;
     p.ostride = o.niter + 3;
     p.cap = o.niter + 2;
     p.extent = (o.niter + 99) / 100;
     p.jcap = 2 * (o.niter + 2);
-    p.nchunks = 512;
+    p.nchunks = (int32_t)((op.N + kGemvCh - 1) / kGemvCh);
     ps.nchunks = p.nchunks;
     ps.jcap = p.jcap;
     auto alloc = [&](size_t bytes, void **ptr) -> hipError_t {
@@ -274,10 +386,27 @@ hipError_t pro_init(ProState &ps, const nv_lanczos_opts &o, const Operator &op, 
     if (e == hipSuccess) e = alloc(8 * sizeof(int64_t), (void **)&p.meta);
     if (e == hipSuccess) e = alloc(p.jcap * sizeof(double), (void **)&p.dots_a);
     if (e == hipSuccess) e = alloc(p.jcap * sizeof(double), (void **)&p.dots_b);
-    const int64_t nw = (int64_t)p.nchunks * (kBlock / 64);
-    if (e == hipSuccess) e = alloc((size_t)p.jcap * nw * 2 * sizeof(double), (void **)&p.dpart);
+    if (e == hipSuccess) e = alloc(p.cap * sizeof(int64_t), (void **)&p.roff);
+    if (e == hipSuccess) e = alloc((size_t)p.jcap * p.nchunks * 2 * sizeof(double), (void **)&p.dpart);
     if (e != hipSuccess) return e;
     e = hipMemsetAsync(p.omega, 0, 2 * p.ostride * sizeof(double), s);
+    if (e == hipSuccess) e = hipMemsetAsync(p.lo, 0, p.cap * sizeof(int64_t), s);
+    if (e == hipSuccess) e = hipMemsetAsync(p.ro, 0, p.cap * sizeof(int64_t), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pro_init_kernel, dim3(1), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t pro_reset(ProState &ps, const nv_lanczos_opts &o, hipStream_t s) {
+    ProDev &p = ps.d;
+    p.chkorthog = chk_period(o);
+    p.seed = o.seed;
+    p.noise = o.noise_scale;
+    p.eps = o.pro_eps > 0 ? o.pro_eps : PRO_EPS64;
+    p.seps = sqrt(p.eps);
+    p.eta = pow(p.eps, 0.75);
+    p.epsn = p.eps * (double)p.epsn_scale;
+    hipError_t e = hipMemsetAsync(p.omega, 0, 2 * p.ostride * sizeof(double), s);
     if (e == hipSuccess) e = hipMemsetAsync(p.lo, 0, p.cap * sizeof(int64_t), s);
     if (e == hipSuccess) e = hipMemsetAsync(p.ro, 0, p.cap * sizeof(int64_t), s);
     if (e != hipSuccess) return e;
@@ -288,13 +417,14 @@ hipError_t pro_init(ProState &ps, const nv_lanczos_opts &o, const Operator &op, 
 static hipError_t multidot(const ProState &ps, int64_t N, const LzState &st, const int64_t *jlist,
                            const int32_t *count_dev, int32_t count_host, int a_cur, int b_res, hipStream_t s) {
     const ProDev &p = ps.d;
-    hipLaunchKernelGGL(pro_multidot_kernel, dim3(p.nchunks), dim3(kBlock), 0, s, N, st, jlist, count_dev,
-                       count_host, (const double *)nullptr, a_cur, b_res, p.dpart, p.nchunks);
-    const int64_t nw = (int64_t)p.nchunks * (kBlock / 64);
-    hipLaunchKernelGGL(pro_reduce_kernel, dim3(64), dim3(64), 0, s, count_dev, count_host, p.dpart, nw, p.dots_a,
-                       p.dots_b);
+    hipLaunchKernelGGL(pro_multidot_kernel, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev, count_host,
+                       a_cur, b_res, p.dpart);
+    hipLaunchKernelGGL(pro_reduce_kernel, dim3(256), dim3(kBlock), 0, s, N, count_dev, count_host, p.dpart,
+                       p.dots_a, p.dots_b);
     return hipGetLastError();
 }
+
+size_t pro_decide_lds(int64_t niter) { return (size_t)(((niter + 3 + 15) & ~int64_t(15)) + 4 * (niter + 3)); }
 
 hipError_t pro_set_omega(ProState &ps, const NvOpDev &dev, const LzState &st, int64_t it, hipStream_t s) {
     // `it` sizes the exact-overlap passes; the kernels read the iteration
@@ -323,7 +453,7 @@ hipError_t pro_set_omega(ProState &ps, const NvOpDev &dev, const LzState &st, in
 
 hipError_t pro_partial_orthog(ProState &ps, const NvOpDev &dev, const LzState &st, hipStream_t s) {
     const ProDev &p = ps.d;
-    hipLaunchKernelGGL(pro_decide_kernel, dim3(1), dim3(64), 0, s, p, st);
+    hipLaunchKernelGGL(pro_decide_kernel, dim3(1), dim3(kBlock), pro_decide_lds(p.niter), s, p, st);
     hipError_t e = multidot(ps, dev.N, st, p.jlist, p.njl, 0, 0, 1, s);
     if (e != hipSuccess) return e;
     const int64_t N = dev.N;

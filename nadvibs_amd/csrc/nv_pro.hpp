@@ -41,8 +41,12 @@ struct ProDev {
     int32_t exact = 0;
     uint64_t seed = 0;
     double noise = 1.0;
+    double eps = 0.0;            // unit roundoff of the recurrence
+    double seps = 0.0, eta = 0.0;  // sqrt(eps), eps^(3/4)
     double epsn = 0.0;           // epsilon * nstates * dimen
+    double epsn_scale = 0.0;     // nstates * dimen
     int64_t *lo = nullptr, *ro = nullptr;  // ranges [cap]
+    int64_t *roff = nullptr;     // first list entry of each range [cap]
     int64_t cap = 0;             // range slots
     int64_t extent = 0;          // ceil(niter/100): the reference's range array size
     int64_t *jlist = nullptr;    // vectors to orthogonalise against, in order
@@ -62,6 +66,8 @@ struct ProState {
 
 hipError_t pro_init(ProState &p, const nv_lanczos_opts &o, const Operator &op, const NvOpDev &dev,
                     const LzState &st, std::vector<void *> &allocs, hipStream_t s);
+size_t pro_decide_lds(int64_t niter);
+hipError_t pro_reset(ProState &p, const nv_lanczos_opts &o, hipStream_t s);
 hipError_t pro_set_omega(ProState &p, const NvOpDev &dev, const LzState &st, int64_t it, hipStream_t s);
 hipError_t pro_partial_orthog(ProState &p, const NvOpDev &dev, const LzState &st, hipStream_t s);
 hipError_t pro_get_omega(const ProState &p, double *omega, int64_t n);

@@ -106,9 +106,9 @@ class Engine:
         return q
 
     # -- Lanczos ---------------------------------------------------------------
-    def lanczos_init(self, niter, reorthog=0, prec="f64", chkorthog=100, seed=12345, noise=1.0):
+    def lanczos_init(self, niter, reorthog=0, prec="f64", chkorthog=100, seed=12345, noise=1.0, pro_eps=0.0):
         o = _lib.NvLanczosOpts(niter=niter, reorthog=reorthog, prec=_lib.PREC[prec], chkorthog=chkorthog,
-                               seed=seed, noise_scale=noise)
+                               seed=seed, noise_scale=noise, pro_eps=pro_eps)
         self._check(self.lib.nv_lanczos_init(self.h, ctypes.byref(o)))
         self.niter = niter
         self.reorthog = reorthog
@@ -161,6 +161,15 @@ class Engine:
         self._check(self.lib.nv_ritz(self.h, n, dptr(alpha), dptr(beta), dptr(ev), dptr(it), dptr(bj),
                                      sg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return ev, it, bj, sg
+
+    def profile(self, on=True):
+        self._check(self.lib.nv_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        """(milliseconds, launches) of the timed H*v launches since the last read."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        self._check(self.lib.nv_profile_read(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def save_vectors(self, path):
         self._check(self.lib.nv_save_vectors(self.h, os.fsencode(path)))

@@ -298,6 +298,17 @@ static double o_h(const OOp *op, int p, long nmin, int adelta) {
     return op->homatel[op->iordr[adelta / 2 + 1][p] + nmin];
 }
 
+static int g_off32 = 0;
+static double g_eps_pro = 0.0;
+
+/* Unit roundoff used by the omega recurrence and its thresholds (0: the
+ * reference's 2^-53). */
+void or_set_eps_pro(double e) { g_eps_pro = e; }
+/* Emulation switch: the off-diagonal (vibrational) part of H*v with the
+ * input rounded to float and float accumulation, the diagonal in double --
+ * the precision split of the library's mixed mode. */
+void or_set_off32(int on) { g_off32 = on; }
+
 /* y = H x, Source/top.f90:672-818 */
 void or_hv(const OOp *op, const double *x, double *y) {
     const int ns = op->ns, M = op->M;
@@ -339,6 +350,12 @@ void or_hv(const OOp *op, const double *x, double *y) {
         }
     }
     /* off-diagonal terms: terms -> steps -> rows -> pairs (:758-808) */
+    float *yf = NULL, *xf = NULL;
+    if (g_off32) {
+        yf = (float *)calloc(op->N, sizeof(float));
+        xf = (float *)malloc(op->N * sizeof(float));
+        for (long i = 0; i < op->N; ++i) xf[i] = (float)x[i];
+    }
     for (int t = 0; t < op->nterms; ++t) {
         const OTerm *T = &op->terms[t];
         if (T->ndir == 0) continue;
@@ -359,6 +376,11 @@ void or_hv(const OOp *op, const double *x, double *y) {
                         d1 = d1 * o_h(op, T->pow[l], n < n2 ? n : n2, abs(d[l]));
                     }
                     if (!ok) continue;
+                    if (g_off32) {
+                        for (int q = 0; q < T->npairs; ++q)
+                            yf[T->prow[q] * D + r] += xf[T->pcol[q] * D + r + stride] * (float)T->pcoef[q] * (float)d1;
+                        continue;
+                    }
                     for (int q = 0; q < T->npairs; ++q)
                         y[T->prow[q] * D + r] += x[T->pcol[q] * D + r + stride] * T->pcoef[q] * d1;
                 }
@@ -368,6 +390,11 @@ void or_hv(const OOp *op, const double *x, double *y) {
             if (l < 0) break;
             d[l] += 2;
         }
+    }
+    if (g_off32) {
+        for (long i = 0; i < op->N; ++i) y[i] += (double)yf[i];
+        free(yf);
+        free(xf);
     }
 }
 
@@ -434,7 +461,7 @@ int or_lanczos(const OOp *op, long niter, int reorthog, long chkorthog, uint64_t
                const double *start, double *alpha, double *beta, double *omega_out, double *archive_in,
                long *stats) {
     const long N = op->N;
-    const double eps = 1.1102230246251565e-16;  /* determine_epsilon: 2^-53 */
+    const double eps = g_eps_pro > 0 ? g_eps_pro : 1.1102230246251565e-16;  /* determine_epsilon: 2^-53 */
     const double eta = pow(eps, 0.75);
     const int orthog = reorthog > 0, exact = reorthog > 1;
     double *q1 = (double *)calloc(N, sizeof(double));

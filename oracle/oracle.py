@@ -52,6 +52,8 @@ def lib():
         l.or_lanczos.restype = ctypes.c_int
         l.or_lanczos.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_int, ctypes.c_long, ctypes.c_uint64,
                                  ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long)]
+        l.or_set_off32.argtypes = [ctypes.c_int]
+        l.or_set_eps_pro.argtypes = [ctypes.c_double]
         l.or_givens.argtypes = [ctypes.c_long, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int), _dp]
         _lib = l
     return _lib
@@ -109,7 +111,18 @@ class Oracle:
         lib().or_term_summary(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
         return out
 
-    def lanczos(self, niter, reorthog=0, chkorthog=100, seed=12345, noise=1.0, start=None, keep_basis=False):
+    def lanczos(self, niter, reorthog=0, chkorthog=100, seed=12345, noise=1.0, start=None, keep_basis=False,
+                mixed=False):
+        """mixed=True emulates the library's mixed mode: off-diagonal part of
+        H*v from a float copy with float accumulation, exact overlap checks
+        every min(chkorthog, 10) iterations."""
+        if mixed:
+            lib().or_set_off32(1)
+            chkorthog = min(chkorthog, 10)
+            try:
+                return self.lanczos(niter, reorthog, chkorthog, seed, noise, start, keep_basis, mixed=False)
+            finally:
+                lib().or_set_off32(0)
         a = np.zeros(niter)
         b = np.zeros(niter + 1)
         om = np.zeros(2 * (niter + 1))
@@ -124,6 +137,17 @@ class Oracle:
         if keep_basis:
             out["basis"] = arch.reshape(niter + 1, self.n)[1:]
         return out
+
+
+def set_off32(on):
+    """Emulates the library's mixed mode in the oracle: off-diagonal part of
+    H*v from a float copy of x with float accumulation."""
+    lib().or_set_off32(1 if on else 0)
+
+
+def set_eps_pro(eps):
+    """Unit roundoff of the omega recurrence (0 = the reference's 2^-53)."""
+    lib().or_set_eps_pro(eps)
 
 
 def givens(alpha, beta, vectors=False):

@@ -202,3 +202,31 @@ def test_exact_omega_mode_keeps_orthogonality():
     Q = r["basis"]
     assert r["events"] > 0
     assert np.max(np.abs(Q @ Q.T - np.eye(100))) < 1e-6
+
+
+def test_mixed_mode_needs_frequent_exact_checks():
+    """The mixed mode's rule, pinned on the emulation (off-diagonal H*v in
+    float, diagonal in double): with the reference's check period of 100 the
+    omega model, which assumes FP64 rounding, lets orthogonality collapse
+    (ghost 2-cycles, then overflow); with checks every 10 iterations the
+    converged eigenvalues stay within 1e-3 cm^-1 of the FP64 run."""
+    from oracle.oracle import set_off32
+    p = workload("cfg1")
+    o = Oracle(p)
+    set_off32(1)
+    try:
+        bad = o.lanczos(400, reorthog=1, chkorthog=100, seed=5, keep_basis=True)
+    finally:
+        set_off32(0)
+    Q = np.nan_to_num(bad["basis"][:300])
+    assert np.max(np.abs(Q @ Q.T - np.eye(300))) > 0.5
+    r64 = o.lanczos(1000, reorthog=1, chkorthog=100, seed=5)
+    r32 = o.lanczos(1000, reorthog=1, chkorthog=100, seed=5, mixed=True)
+    assert np.all(np.isfinite(r32["beta"])) and r32["beta"].max() < 1.0
+    ev64, it64, bj64, _ = givens(r64["alpha"], r64["beta"])
+    ev32, it32, _, _ = givens(r32["alpha"], r32["beta"])
+    conv = np.nonzero(bj64 < 1e-9)[0]
+    assert conv.size > 100
+    j = np.argmin(np.abs(ev32[None, :] - ev64[conv][:, None]), axis=1)
+    assert np.max(np.abs(ev32[j] - ev64[conv])) * 219474.6313705 < 1e-3
+    assert np.max(np.abs(it32[j] - it64[conv])) < 1e-6
