@@ -35,13 +35,14 @@ def run_job(eng, niter, prec, reorthog, chkorthog, seed):
 
 
 def timed(eng, args, prec, sync):
+    """W untimed jobs, K timed jobs (captured iteration graphs), then one
+    more job with every H*v launch bracketed by HIP events on the library's
+    stream for the kernel's average duration."""
     for k in range(args.warmup):
         t0 = time.perf_counter()
         run_job(eng, args.niter, prec, args.reorthog, args.chkorthog, args.seed)
         print(f"# {prec} warmup {k + 1}/{args.warmup}: {time.perf_counter() - t0:.2f} s", file=sys.stderr,
               flush=True)
-    eng.profile(True)
-    eng.profile_read()
     sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -50,9 +51,14 @@ def timed(eng, args, prec, sync):
               flush=True)
     sync()
     dt = time.perf_counter() - t0
+    pro = eng.pro_stats()
+    eng.profile(True)
+    eng.profile_read()
+    run_job(eng, args.niter, prec, args.reorthog, args.chkorthog, args.seed)
     hv_ms, hv_n = eng.profile_read()
     eng.profile(False)
-    return dt, hv_ms, hv_n, out, eng.pro_stats()
+    print(f"# {prec} profiled job: H*v {hv_ms / max(hv_n, 1) * 1e3:.1f} us x {hv_n}", file=sys.stderr, flush=True)
+    return dt, hv_ms, hv_n, out, pro
 
 
 def compare(ev64, it64, bj64, ev, it, bj, tol_conv=1e-9):

@@ -35,7 +35,9 @@ struct nv_ctx {
     std::vector<void *> lz_allocs;
     nv::ProState pro{};
 
-    // graph of one iteration
+    // graphs of one iteration: [0] plain, [1] with re-orthogonalisation,
+    // [2] re-orthogonalisation with the exact overlap check
+    hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t graph = nullptr;
 
     // H*v timing
@@ -83,7 +85,13 @@ void free_list(std::vector<void *> &l) {
     l.clear();
 }
 
+void release_graphs(nv_ctx *c) {
+    for (auto &g : c->graphs)
+        if (g) { hipGraphExecDestroy(g); g = nullptr; }
+}
+
 void release_lanczos(nv_ctx *c) {
+    release_graphs(c);
     if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
     free_list(c->lz_allocs);
     c->st = nv::LzState{};
@@ -133,6 +141,32 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
         g_create_error = std::string("device setup failed: ") + hipGetErrorString(e);
         free_list(c->allocs);
         if (c->stream) hipStreamDestroy(c->stream);
+        delete c;
+        return NV_ERR_HIP;
+    }
+    // state-diagonal part of H, and whether D couples different states
+    d.offstate = 0;
+    for (int64_t a = 0; a < op.ns; ++a)
+        for (int64_t b = 0; b < op.ns; ++b) {
+            if (a == b) continue;
+            if (op.dconst[a * op.ns + b] != 0.0) d.offstate = 1;
+            for (size_t t = 0; t < op.dev_dterms.size(); ++t)
+                if (op.dcmat[t * op.ns * op.ns + a * op.ns + b] != 0.0) d.offstate = 1;
+        }
+    if (e == hipSuccess) {
+        void *p = nullptr;
+        e = hipMalloc(&p, (size_t)op.N * sizeof(double));
+        if (e == hipSuccess) {
+            c->allocs.push_back(p);
+            d.dg = (const double *)p;
+            e = nv::launch_diag_precompute(d, (double *)p, c->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (e != hipSuccess) {
+        g_create_error = std::string("device setup failed: ") + hipGetErrorString(e);
+        free_list(c->allocs);
+        hipStreamDestroy(c->stream);
         delete c;
         return NV_ERR_HIP;
     }
@@ -261,8 +295,8 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     if (!c || !o) return NV_ERR_ARG;
     if (o->niter < 1) return fail(c, NV_ERR_ARG, "niter must be >= 1");
     if (!valid_prec(o->prec)) return fail(c, NV_ERR_ARG, "unknown precision");
-    if (o->reorthog > 0 && nv::pro_decide_lds(o->niter) > 64 * 1024)
-        return fail(c, NV_ERR_ARG, "re-orthogonalisation supports up to 13000 iterations");
+    if (o->reorthog > 0 && nv::pro_decide_lds(o->niter) != 0)
+        return fail(c, NV_ERR_ARG, "re-orthogonalisation supports up to 32768 iterations");
     NV_HIP(c, hipSetDevice(c->device));
     NV_HIP(c, hipStreamSynchronize(c->stream));
     const int64_t N = c->op.N;
@@ -271,6 +305,8 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     const bool same = c->lz_ready && c->opts.niter == o->niter && c->opts.prec == o->prec &&
                       (c->opts.reorthog > 0) == keep && (c->opts.reorthog > 1) == (o->reorthog > 1);
     nv::LzState &st = c->st;
+    // captured iteration graphs hold the options by value
+    if (!same || std::memcmp(&c->opts, o, sizeof(*o)) != 0) release_graphs(c);
     if (!same) {
         release_lanczos(c);
         st.nslots = keep ? o->niter + 2 : 3;
@@ -387,7 +423,8 @@ int32_t nv_set_omega(nv_ctx *c, int64_t it) {
     int32_t rc = check_iter(c, it);
     if (rc) return rc;
     if (c->opts.reorthog <= 0 || it <= 1) return NV_OK;
-    NV_HIP(c, nv::pro_set_omega(c->pro, c->dev, c->st, it, c->stream));
+    const int check = (it % nv::pro_check_period(c->pro)) == 0;
+    NV_HIP(c, nv::pro_set_omega(c->pro, c->dev, c->st, check, c->stream));
     return NV_OK;
 }
 
@@ -399,20 +436,65 @@ int32_t nv_partial_orthog(nv_ctx *c, int64_t it) {
     return NV_OK;
 }
 
+// The kernels of one iteration, in the reference's order; the iteration
+// number is advanced on the device so that a captured sequence replays for
+// every iteration.
+static hipError_t enqueue_iteration(nv_ctx *c, int pro, int check) {
+    hipError_t e = nv::launch_set_iter(c->d_iter, 0, 1, c->stream);
+    if (e == hipSuccess) e = nv::launch_normalize(c->dev, c->opts.prec, c->st, c->stream);
+    if (e == hipSuccess) e = nv::launch_hv_iter(c->dev, c->opts.prec, c->st, c->stream);
+    if (e == hipSuccess) e = nv::launch_lanczos_step(c->dev, c->opts.prec, c->st, c->stream);
+    if (pro) {
+        if (e == hipSuccess) e = nv::pro_set_omega(c->pro, c->dev, c->st, check, c->stream);
+        if (e == hipSuccess) e = nv::pro_partial_orthog(c->pro, c->dev, c->st, c->stream);
+    }
+    return e;
+}
+
+static hipError_t iteration_graph(nv_ctx *c, int kind, hipGraphExec_t *out) {
+    if (c->graphs[kind]) {
+        *out = c->graphs[kind];
+        return hipSuccess;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t e2 = enqueue_iteration(c, kind >= 1, kind == 2);
+    e = hipStreamEndCapture(c->stream, &g);
+    if (e2 != hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&c->graphs[kind], g, nullptr, nullptr, 0);
+    if (g) hipGraphDestroy(g);
+    *out = c->graphs[kind];
+    return e;
+}
+
 int32_t nv_lanczos_run(nv_ctx *c, int64_t iiter, int64_t niter) {
     if (!c) return NV_ERR_ARG;
     if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
     if (iiter != c->next_update) return fail(c, NV_ERR_STATE, "run must start at the next iteration");
     if (niter > c->opts.niter) return fail(c, NV_ERR_ARG, "niter beyond the initialised run");
+    NV_HIP(c, hipSetDevice(c->device));
+    const bool orthog = c->opts.reorthog > 0;
     for (int64_t i = iiter; i <= niter; ++i) {
-        int32_t rc = nv_update_lanczos(c, i);
-        if (!rc) rc = nv_hv_iter(c, i);
-        if (!rc) rc = nv_lanczos_step(c, i);
-        if (!rc && c->opts.reorthog > 0 && i > 1) {
-            rc = nv_set_omega(c, i);
-            if (!rc) rc = nv_partial_orthog(c, i);
+        // the first iteration and profiled runs go through the per-call path
+        if (i == 1 || c->prof) {
+            int32_t rc = nv_update_lanczos(c, i);
+            if (!rc) rc = nv_hv_iter(c, i);
+            if (!rc) rc = nv_lanczos_step(c, i);
+            if (!rc && orthog && i > 1) {
+                rc = nv_set_omega(c, i);
+                if (!rc) rc = nv_partial_orthog(c, i);
+            }
+            if (rc) return rc;
+            continue;
         }
-        if (rc) return rc;
+        int kind = 0;
+        if (orthog) kind = (c->opts.reorthog == 1 && i % nv::pro_check_period(c->pro) == 0) ? 2 : 1;
+        hipGraphExec_t ge = nullptr;
+        NV_HIP(c, iteration_graph(c, kind, &ge));
+        NV_HIP(c, hipGraphLaunch(ge, c->stream));
+        c->next_update = i + 1;
+        c->last_iter = i;
     }
     return NV_OK;
 }

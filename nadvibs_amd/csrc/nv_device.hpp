@@ -49,4 +49,6 @@ struct NvOpDev {
     const double *dconst;  // [ns][ns] constant block matrix, masked
     const double *omega;   // [M]
     const int32_t *nfunc;  // [M]
+    const double *dg;      // [ns][D] state-diagonal part E(n) + D_ss(n), precomputed
+    int32_t offstate;      // 1 if D couples different states (then computed per row)
 };

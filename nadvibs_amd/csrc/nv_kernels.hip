@@ -115,16 +115,19 @@ __device__ __forceinline__ double digits_energy(const NvOpDev &op, int r, uint8_
     return e;
 }
 
-// out[s] = E v[s] + sum_s' D_ss'(r) v[s'] in FP64 (Source/top.f90:712-756).
+// The vibrationally diagonal part of H (Source/top.f90:712-756) splits into
+// the state-diagonal E(n) + D_ss(n), precomputed once into op.dg, and the
+// inter-state part D_ss'(n), s != s', evaluated per row when present.
 template <int NS>
-__device__ __forceinline__ void diag_apply(const NvOpDev &op, double E, const uint8_t *dig, int tid,
-                                           const double (&v)[NS], double (&out)[NS]) {
+__device__ __forceinline__ void diag_offstate(const NvOpDev &op, const uint8_t *dig, int tid,
+                                              const double (&v)[NS], double (&out)[NS]) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        double a = E * v[s];
+        double a = 0.0;
 #pragma unroll
-        for (int s2 = 0; s2 < NS; ++s2) a += op.dconst[s * NS + s2] * v[s2];
-        out[s] = a;
+        for (int s2 = 0; s2 < NS; ++s2)
+            if (s2 != s) a += op.dconst[s * NS + s2] * v[s2];
+        out[s] += a;
     }
     for (int t = 0; t < op.ndterms; ++t) {
         const NvDiagTerm dt = op.dterms[t];
@@ -137,10 +140,47 @@ __device__ __forceinline__ void diag_apply(const NvOpDev &op, double E, const ui
         for (int s = 0; s < NS; ++s) {
             double a = 0.0;
 #pragma unroll
-            for (int s2 = 0; s2 < NS; ++s2) a += C[s * NS + s2] * v[s2];
+            for (int s2 = 0; s2 < NS; ++s2)
+                if (s2 != s) a += C[s * NS + s2] * v[s2];
             out[s] += h * a;
         }
     }
+}
+
+// out[s] = sum_s' (E + D)(n)_{ss'} v[s'] in FP64.  dig is only read when
+// states are coupled through D.
+template <int NS>
+__device__ __forceinline__ void diag_apply(const NvOpDev &op, int r, const uint8_t *dig, int tid,
+                                           const double (&v)[NS], double (&out)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) out[s] = op.dg[(size_t)s * op.D + r] * v[s];
+    if (op.offstate) diag_offstate<NS>(op, dig, tid, v, out);
+}
+
+// dg[s][r] = E(r) + D_ss(r): harmonic energy, constant term and the Delta=0
+// part of the all-even monomials of the diagonal blocks.
+template <int NS>
+__global__ __launch_bounds__(kBlock) void diag_precompute_kernel(NvOpDev op, double *dg) {
+    __shared__ uint8_t dig[NV_MAXM * kBlock];
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x * kBlock + tid;
+    if (r >= op.D) return;
+    const double E = digits_energy(op, r, dig, tid);
+    double d[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) d[s] = E + op.dconst[s * NS + s];
+    for (int t = 0; t < op.ndterms; ++t) {
+        const NvDiagTerm dt = op.dterms[t];
+        double h = 1.0;
+#pragma unroll
+        for (int j = 0; j < NV_MAXF; ++j)
+            if (j < dt.nf) h *= op.ftab64[dt.code[j] * op.nmax + dig[dt.mode[j] * kBlock + tid]];
+        const double *C = op.dcmat + (size_t)t * NS * NS;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) d[s] += h * C[s * NS + s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) dg[(size_t)s * op.D + r] = d[s];
 }
 
 // ------------------------------------------------------------------------
@@ -204,8 +244,7 @@ __global__ __launch_bounds__(kBlock) void hv_full_kernel(NvOpDev op, const doubl
     const int tid = threadIdx.x;
     const int r = blockIdx.x * kBlock + tid;
     const bool live = r < op.D;
-    double E = 0.0;
-    if (live) E = digits_energy(op, r, dig, tid);
+    if (live) digits_energy(op, r, dig, tid);
     __syncthreads();
     if (!live) return;
     TAcc acc[NS];
@@ -213,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void hv_full_kernel(NvOpDev op, const doubl
     double v[NS], dv[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) v[s] = x[(size_t)s * op.D + r];
-    diag_apply<NS>(op, E, dig, tid, v, dv);
+    diag_apply<NS>(op, r, dig, tid, v, dv);
 #pragma unroll
     for (int s = 0; s < NS; ++s) y[(size_t)s * op.D + r] = dv[s] + (double)acc[s];
 }
@@ -231,8 +270,7 @@ __global__ __launch_bounds__(kBlock) void hv_iter_kernel(NvOpDev op, LzState st)
     const int64_t it = *st.iter;
     const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;
     const TIn *x = st.vlo ? reinterpret_cast<const TIn *>(st.vlo) : reinterpret_cast<const TIn *>(q);
-    double E = 0.0;
-    if (live) E = digits_energy(op, r, dig, tid);
+    if (live) digits_energy(op, r, dig, tid);
     __syncthreads();
     double part = 0.0;
     if (live) {
@@ -241,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void hv_iter_kernel(NvOpDev op, LzState st)
         double v[NS], dv[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
-        diag_apply<NS>(op, E, dig, tid, v, dv);
+        diag_apply<NS>(op, r, dig, tid, v, dv);
         TAcc *y = reinterpret_cast<TAcc *>(st.y);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
@@ -258,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void hv_iter_kernel(NvOpDev op, LzState st)
 // stored in slot i+1, beta(i) = |r|.
 template <int NS, typename TAcc>
 __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzState st) {
-    __shared__ uint8_t dig[NV_MAXM * kBlock];
+    extern __shared__ __align__(16) unsigned char dig[];  // M*kBlock bytes when states couple through D
     const int tid = threadIdx.x;
     const int r = blockIdx.x * kBlock + tid;
     const int64_t it = *st.iter;
@@ -269,11 +307,11 @@ __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzStat
     const double b = it > 1 ? st.beta[it - 1] : 0.0;
     double part = 0.0;
     if (r < op.D) {
-        const double E = digits_energy(op, r, dig, tid);
+        if (op.offstate) digits_energy(op, r, dig, tid);
         double v[NS], dv[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
-        diag_apply<NS>(op, E, dig, tid, v, dv);
+        diag_apply<NS>(op, r, dig, tid, v, dv);
         const TAcc *y = reinterpret_cast<const TAcc *>(st.y);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
@@ -368,10 +406,11 @@ static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hip
 
 template <int NS>
 static hipError_t step_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    const size_t lds = op.offstate ? (size_t)op.M * kBlock : 0;
     if (prec == 0)
-        hipLaunchKernelGGL((lanczos_step_kernel<NS, double>), dim3(hv_blocks(op)), dim3(kBlock), 0, s, op, st);
+        hipLaunchKernelGGL((lanczos_step_kernel<NS, double>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
     else
-        hipLaunchKernelGGL((lanczos_step_kernel<NS, float>), dim3(hv_blocks(op)), dim3(kBlock), 0, s, op, st);
+        hipLaunchKernelGGL((lanczos_step_kernel<NS, float>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
     return hipGetLastError();
 }
 
@@ -389,6 +428,16 @@ static hipError_t step_ns(const NvOpDev &op, int prec, const LzState &st, hipStr
     case 10: return fn<10>(__VA_ARGS__);           \
     default: return hipErrorInvalidValue;          \
     }
+
+template <int NS>
+static hipError_t diag_ns(const NvOpDev &op, double *dg, hipStream_t s) {
+    hipLaunchKernelGGL((diag_precompute_kernel<NS>), dim3(hv_blocks(op)), dim3(kBlock), 0, s, op, dg);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_precompute(const NvOpDev &op, double *dg, hipStream_t s) {
+    NV_DISPATCH_NS(diag_ns, op, dg, s)
+}
 
 hipError_t launch_hv_full(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s) {
     NV_DISPATCH_NS(hv_full_ns, op, prec, x, y, s)

@@ -30,6 +30,8 @@ enum { kCntAlpha = 0, kCntBeta = 1, kCntPro = 2, kCntChk = 3, kCntMisc = 4, kNum
 size_t hv_lds_bytes(const NvOpDev &op, int prec);
 int hv_blocks(const NvOpDev &op);
 
+// dg[s][r] = E(r) + D_ss(r) (once per operator).
+hipError_t launch_diag_precompute(const NvOpDev &op, double *dg, hipStream_t s);
 // y = H x, x and y double, full operator (standalone H*v).
 hipError_t launch_hv_full(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s);
 // Converts q_i to the input precision of H*v.

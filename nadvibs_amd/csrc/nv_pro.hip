@@ -70,12 +70,17 @@ constexpr int kGemvG = 8;
 constexpr int kGemvE = 8;
 constexpr int kGemvCh = kBlock * kGemvE;
 
+// count: *count_dev, or iter + count_off when count_dev is null.
+__device__ __forceinline__ int32_t dot_count(const LzState &st, const int32_t *count_dev, int32_t count_off) {
+    return count_dev ? *count_dev : (int32_t)(*st.iter + count_off);
+}
+
 __global__ __launch_bounds__(kBlock) void pro_multidot_kernel(int64_t N, LzState st, const int64_t *jlist,
-                                                              const int32_t *count_dev, int32_t count_host,
+                                                              const int32_t *count_dev, int32_t count_off,
                                                               int a_cur, int b_res, double *part) {
     __shared__ double red[kBlock / 64][2 * kGemvG];
     const int64_t it = *st.iter;
-    const int32_t count = count_dev ? *count_dev : count_host;
+    const int32_t count = dot_count(st, count_dev, count_off);
     if (count <= 0) return;
     const double *a = a_cur ? st.V + (it % st.nslots) * N : nullptr;
     const double *b = b_res ? st.V + ((it + 1) % st.nslots) * N : nullptr;
@@ -138,9 +143,10 @@ __global__ __launch_bounds__(kBlock) void pro_multidot_kernel(int64_t N, LzState
 }
 
 // One wave per dot: sums its chunk partials in chunk order.
-__global__ __launch_bounds__(kBlock) void pro_reduce_kernel(int64_t N, const int32_t *count_dev, int32_t count_host,
-                                                            const double *part, double *out_a, double *out_b) {
-    const int32_t count = count_dev ? *count_dev : count_host;
+__global__ __launch_bounds__(kBlock) void pro_reduce_kernel(int64_t N, LzState st, const int32_t *count_dev,
+                                                            int32_t count_off, const double *part, double *out_a,
+                                                            double *out_b) {
+    const int32_t count = dot_count(st, count_dev, count_off);
     const int64_t nchunks = (N + kGemvCh - 1) / kGemvCh;
     const int lane = threadIdx.x & 63;
     const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -185,103 +191,168 @@ __global__ void pro_exact_kernel(ProDev p, LzState st) {
         o2[i] = i == 0 ? 0.0 : p.dots_b[i - 1];
 }
 
-// Range detection of partial_orthog (Source/top.f90:924-970).  The reference
-// walks left and right from every index with |omega| >= sqrt(eps) to the
-// nearest |omega| < eps^(3/4); the same edges come from one backward scan
-// (next small index) and the running last small index, so the single-thread
-// part is O(iter) over LDS.  The list of vectors and the omega resets are
-// then written by the whole block, range after range so that an index in two
-// ranges keeps the draw of the later one, as in the reference.
-__global__ __launch_bounds__(kBlock) void pro_decide_kernel(ProDev p, LzState st) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int64_t sh_n, sh_base, sh_nr;
-    const int64_t it = *st.iter;
-    double *o2 = p.omega + p.ostride;
-    uint8_t *cls = lds;                                            // [it+2]
-    int32_t *nxt = reinterpret_cast<int32_t *>(lds + ((it + 2 + 15) & ~int64_t(15)));  // [it+2]
-    const double seps = p.seps, eta = p.eta;
-    for (int64_t i = threadIdx.x; i <= it; i += blockDim.x) {
-        const double w = fabs(o2[i]);
-        cls[i] = (uint8_t)((w < eta ? 1 : 0) | (w >= seps ? 2 : 0));
+// Range detection of partial_orthog (Source/top.f90:924-970), block-parallel.
+// The reference walks from every index with |omega| >= sqrt(eps) left and
+// right to the nearest indices with |omega| < eps^(3/4) and keeps a range when
+// both ends differ from the last kept one.  Both ends are monotone in the
+// index, so the kept ranges are exactly the gaps between consecutive "small"
+// indices (with 0 and iter+1 as sentinels, ends clipped to [1, iter]) that
+// hold at least one "large" index: two scans and a compaction find them.
+// Then the reference's extra range [1,1], the list of vectors and the omega
+// resets (a vector on the boundary of two ranges keeps the later draw, as the
+// reference's sequential loop leaves it).
+constexpr int kDecide = 1024;
+
+__device__ int64_t scan_excl(int64_t v, int64_t *total) {
+    __shared__ int64_t wsum[kDecide / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int64_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t firststep = p.meta[0], northog = p.meta[1];
-        if (firststep) {
-            int32_t nx = (int32_t)(it + 1);
-            for (int64_t i = it; i >= 1; --i) {
-                if (cls[i] & 1) nx = (int32_t)i;
-                nxt[i] = nx;
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int64_t before = 0, tot = 0;
+    for (int w = 0; w < kDecide / 64; ++w) {
+        if (w < wid) before += wsum[w];
+        tot += wsum[w];
+    }
+    *total = tot;
+    return before + x - v;
+}
+
+__global__ __launch_bounds__(kDecide) void pro_decide_kernel(ProDev p, LzState st) {
+    __shared__ int64_t sh_nr, sh_n, sh_first;
+    const int64_t it = *st.iter;
+    double *o2 = p.omega + p.ostride;
+    int32_t *S = p.scr;                 // small indices in order
+    uint8_t *act = reinterpret_cast<uint8_t *>(p.scr + (it + 3));  // gap holds a large index
+    const int64_t seg = (it + kDecide - 1) / kDecide;  // indices 1..it, <= 32 per thread
+    const int64_t i0 = 1 + (int64_t)threadIdx.x * seg;
+    const int64_t i1 = i0 + seg - 1 < it ? i0 + seg - 1 : it;
+    if (threadIdx.x == 0) sh_first = p.meta[0];
+    __syncthreads();
+    const bool firststep = sh_first != 0;
+    int64_t tot;
+    if (firststep) {
+        uint32_t smallm = 0, bigm = 0;
+        for (int64_t i = i0; i <= i1; ++i) {
+            const double w = fabs(o2[i]);
+            if (w < p.eta) smallm |= 1u << (i - i0);
+            if (w >= p.seps) bigm |= 1u << (i - i0);
+        }
+        const int64_t sbase = scan_excl(__popc(smallm), &tot);
+        const int64_t m = tot;  // number of small indices; gaps 0..m
+        for (int64_t g = threadIdx.x; g <= m; g += kDecide) act[g] = 0;
+        __syncthreads();
+        int64_t k = sbase;
+        for (int64_t i = i0; i <= i1; ++i) {
+            const uint32_t bit = 1u << (i - i0);
+            if (bigm & bit) act[k] = 1;
+            if (smallm & bit) S[k++] = (int32_t)i;
+        }
+        __syncthreads();
+        // compaction of the active gaps, in order
+        const int64_t gseg = (m + 1 + kDecide - 1) / kDecide;
+        const int64_t g0 = (int64_t)threadIdx.x * gseg;
+        const int64_t g1 = g0 + gseg < m + 1 ? g0 + gseg : m + 1;
+        int64_t cnt = 0;
+        for (int64_t g = g0; g < g1; ++g) cnt += act[g];
+        int64_t r0 = scan_excl(cnt, &tot);
+        for (int64_t g = g0; g < g1; ++g) {
+            if (!act[g]) continue;
+            if (r0 < p.cap) {
+                const int64_t a = g == 0 ? 0 : S[g - 1];
+                const int64_t b = g == m ? it + 1 : S[g];
+                p.lo[r0] = a > 1 ? a : 1;
+                p.ro[r0] = b < it ? b : it;
             }
-            northog = 0;
-            int64_t prev_lo = 0, prev_ro = 0, last_small = 0;
-            for (int64_t i = 1; i <= it; ++i) {
-                if (cls[i] & 1) last_small = i;
-                if (!(cls[i] & 2)) continue;
-                const int64_t l = last_small >= 1 ? last_small : 1;
-                const int64_t r = nxt[i] <= it ? nxt[i] : it;
-                if (northog == 0 || (l != prev_lo && r != prev_ro)) {
-                    if (northog < p.cap) {
-                        p.lo[northog] = l;
-                        p.ro[northog] = r;
-                        ++northog;
-                    } else {
-                        p.meta[5] = 1;
-                    }
-                    prev_lo = l;
-                    prev_ro = r;
-                }
+            ++r0;
+        }
+        int64_t northog = tot < p.cap ? tot : p.cap;
+        __syncthreads();
+        if (northog > 0) {
+            // minval(loindex, mask=loindex>0) over the stored ranges, including
+            // those left from earlier detections, as the reference's array keeps them
+            const int64_t span = northog > p.extent ? northog : p.extent;
+            int64_t mn = INT64_MAX;
+            for (int64_t q = threadIdx.x; q < span && q < p.cap; q += kDecide)
+                if (p.lo[q] > 0 && p.lo[q] < mn) mn = p.lo[q];
+            for (int off = 32; off > 0; off >>= 1) {
+                const int64_t y = __shfl_xor(mn, off, 64);
+                mn = y < mn ? y : mn;
             }
-            if (northog > 0) {
-                // minval(loindex, mask=loindex>0) over the stored ranges,
-                // including those left from earlier detections, as the
-                // reference's array keeps them
-                const int64_t span = northog > p.extent ? northog : p.extent;
-                int64_t mn = INT64_MAX;
-                for (int64_t k = 0; k < span && k < p.cap; ++k)
-                    if (p.lo[k] > 0 && p.lo[k] < mn) mn = p.lo[k];
-                if (mn > 1 && northog < p.cap) {
+            __shared__ int64_t wmin[kDecide / 64];
+            if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = mn;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int64_t v = INT64_MAX;
+                for (int w = 0; w < kDecide / 64; ++w) v = wmin[w] < v ? wmin[w] : v;
+                if (v > 1 && northog < p.cap) {
                     p.lo[northog] = 1;
                     p.ro[northog] = 1;
                     ++northog;
                 }
+                if (tot > p.cap) p.meta[5] = 1;
+                sh_nr = northog;
             }
+        } else if (threadIdx.x == 0) {
+            sh_nr = 0;
         }
-        int64_t n = 0;
-        for (int64_t k = 0; k < northog; ++k) {
-            p.roff[k] = n;
-            n += p.ro[k] - p.lo[k] + 1;
-        }
-        if (n > p.jcap) {
-            p.meta[5] = 1;
-            n = p.jcap;
-        }
-        sh_n = n;
-        sh_nr = northog;
-        sh_base = p.meta[2];
-        if (northog > 0) {
-            p.meta[0] = !firststep;
-            p.meta[2] = sh_base + n;
-            p.meta[3] += 1;
-            p.meta[4] += n;
-        } else {
-            p.meta[0] = firststep;
-        }
-        p.meta[1] = northog;
-        *p.njl = (int32_t)n;
+    } else if (threadIdx.x == 0) {
+        sh_nr = p.meta[1];
     }
     __syncthreads();
-    const int64_t n = sh_n, nr = sh_nr;
-    const uint64_t base = (uint64_t)sh_base;
-    for (int64_t k = 0; k < nr; ++k) {
-        const int64_t off = p.roff[k], lo = p.lo[k], len = p.ro[k] - p.lo[k] + 1;
-        for (int64_t t = threadIdx.x; t < len; t += blockDim.x) {
-            const int64_t c = off + t;
-            if (c >= n) break;
-            p.jlist[c] = lo + t;
-            o2[lo + t] = p.eps * (p.noise * noise_gauss(p.seed, base + (uint64_t)c, 2.0));
+    const int64_t nr = sh_nr;
+    // list offsets
+    {
+        const int64_t rseg = (nr + kDecide - 1) / kDecide;
+        const int64_t k0 = (int64_t)threadIdx.x * rseg;
+        const int64_t k1 = k0 + rseg < nr ? k0 + rseg : nr;
+        int64_t len = 0;
+        for (int64_t q = k0; q < k1; ++q) len += p.ro[q] - p.lo[q] + 1;
+        int64_t off = scan_excl(len, &tot);
+        for (int64_t q = k0; q < k1; ++q) {
+            p.roff[q] = off;
+            off += p.ro[q] - p.lo[q] + 1;
         }
-        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t n = tot;
+            if (n > p.jcap) {
+                p.meta[5] = 1;
+                n = p.jcap;
+            }
+            sh_n = n;
+        }
+    }
+    __syncthreads();
+    const int64_t n = sh_n;
+    const uint64_t base = (uint64_t)p.meta[2];
+    for (int64_t c = threadIdx.x; c < n; c += kDecide) {
+        int64_t lo = 0, hi = nr - 1;  // last range with roff <= c
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (p.roff[mid] <= c) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t k = lo;
+        const int64_t j = p.lo[k] + (c - p.roff[k]);
+        p.jlist[c] = j;
+        const bool shadowed = (j == p.ro[k]) && (k + 1 < nr) && (p.lo[k + 1] == j) && (p.roff[k + 1] < n);
+        if (!shadowed) o2[j] = p.eps * (p.noise * noise_gauss(p.seed, base + (uint64_t)c, 2.0));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (nr > 0) {
+            p.meta[0] = !firststep;
+            p.meta[2] = (int64_t)base + n;
+            p.meta[3] += 1;
+            p.meta[4] += n;
+        }
+        p.meta[1] = nr;
+        *p.njl = (int32_t)n;
     }
 }
 
@@ -387,6 +458,7 @@ hipError_t pro_init(ProState &ps, const nv_lanczos_opts &o, const Operator &op, 
     if (e == hipSuccess) e = alloc(p.jcap * sizeof(double), (void **)&p.dots_a);
     if (e == hipSuccess) e = alloc(p.jcap * sizeof(double), (void **)&p.dots_b);
     if (e == hipSuccess) e = alloc(p.cap * sizeof(int64_t), (void **)&p.roff);
+    if (e == hipSuccess) e = alloc((size_t)2 * (o.niter + 4) * sizeof(int32_t), (void **)&p.scr);
     if (e == hipSuccess) e = alloc((size_t)p.jcap * p.nchunks * 2 * sizeof(double), (void **)&p.dpart);
     if (e != hipSuccess) return e;
     e = hipMemsetAsync(p.omega, 0, 2 * p.ostride * sizeof(double), s);
@@ -415,25 +487,25 @@ hipError_t pro_reset(ProState &ps, const nv_lanczos_opts &o, hipStream_t s) {
 }
 
 static hipError_t multidot(const ProState &ps, int64_t N, const LzState &st, const int64_t *jlist,
-                           const int32_t *count_dev, int32_t count_host, int a_cur, int b_res, hipStream_t s) {
+                           const int32_t *count_dev, int32_t count_off, int a_cur, int b_res, hipStream_t s) {
     const ProDev &p = ps.d;
-    hipLaunchKernelGGL(pro_multidot_kernel, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev, count_host,
+    hipLaunchKernelGGL(pro_multidot_kernel, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev, count_off,
                        a_cur, b_res, p.dpart);
-    hipLaunchKernelGGL(pro_reduce_kernel, dim3(256), dim3(kBlock), 0, s, N, count_dev, count_host, p.dpart,
+    hipLaunchKernelGGL(pro_reduce_kernel, dim3(256), dim3(kBlock), 0, s, N, st, count_dev, count_off, p.dpart,
                        p.dots_a, p.dots_b);
     return hipGetLastError();
 }
 
-size_t pro_decide_lds(int64_t niter) { return (size_t)(((niter + 3 + 15) & ~int64_t(15)) + 4 * (niter + 3)); }
+size_t pro_decide_lds(int64_t niter) { return niter <= 32 * kDecide ? 0 : SIZE_MAX; }
 
-hipError_t pro_set_omega(ProState &ps, const NvOpDev &dev, const LzState &st, int64_t it, hipStream_t s) {
-    // `it` sizes the exact-overlap passes; the kernels read the iteration
-    // from the device.
+int64_t pro_check_period(const ProState &ps) { return ps.d.chkorthog; }
+
+hipError_t pro_set_omega(ProState &ps, const NvOpDev &dev, const LzState &st, int check, hipStream_t s) {
     hipError_t e = hipSuccess;
     const ProDev &p = ps.d;
     if (p.exact) {
-        if (it > p.jcap) return hipErrorInvalidValue;
-        e = multidot(ps, dev.N, st, nullptr, nullptr, (int32_t)it, 0, 1, s);
+        // omega(2, 1:iter) = q3 . q_j (count = iter)
+        e = multidot(ps, dev.N, st, nullptr, nullptr, 0, 0, 1, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(pro_exact_kernel, dim3(16), dim3(kBlock), 0, s, p, st);
         return hipGetLastError();
@@ -441,9 +513,9 @@ hipError_t pro_set_omega(ProState &ps, const NvOpDev &dev, const LzState &st, in
     hipLaunchKernelGGL(pro_recur_kernel, dim3(1), dim3(kBlock), 0, s, p, st);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (it % p.chkorthog == 0 && it > 1) {
-        if (it - 1 > p.jcap) return hipErrorInvalidValue;
-        e = multidot(ps, dev.N, st, nullptr, nullptr, (int32_t)(it - 1), 1, 1, s);
+    if (check) {
+        // q2 . q_j and q3 . q_j for j < iter (count = iter - 1)
+        e = multidot(ps, dev.N, st, nullptr, nullptr, -1, 1, 1, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(pro_check_kernel, dim3(16), dim3(kBlock), 0, s, p, st);
         e = hipGetLastError();
@@ -453,8 +525,8 @@ hipError_t pro_set_omega(ProState &ps, const NvOpDev &dev, const LzState &st, in
 
 hipError_t pro_partial_orthog(ProState &ps, const NvOpDev &dev, const LzState &st, hipStream_t s) {
     const ProDev &p = ps.d;
-    hipLaunchKernelGGL(pro_decide_kernel, dim3(1), dim3(kBlock), pro_decide_lds(p.niter), s, p, st);
-    hipError_t e = multidot(ps, dev.N, st, p.jlist, p.njl, 0, 0, 1, s);
+    hipLaunchKernelGGL(pro_decide_kernel, dim3(1), dim3(kDecide), 0, s, p, st);
+    hipError_t e = multidot(ps, dev.N, st, p.jlist, p.njl, 0, 0, 1, s);  // count = list length
     if (e != hipSuccess) return e;
     const int64_t N = dev.N;
     int g = (int)std::min<int64_t>((N + kBlock - 1) / kBlock, 4096);

@@ -47,6 +47,7 @@ struct ProDev {
     double epsn_scale = 0.0;     // nstates * dimen
     int64_t *lo = nullptr, *ro = nullptr;  // ranges [cap]
     int64_t *roff = nullptr;     // first list entry of each range [cap]
+    int32_t *scr = nullptr;      // detection scratch: small indices, gap flags
     int64_t cap = 0;             // range slots
     int64_t extent = 0;          // ceil(niter/100): the reference's range array size
     int64_t *jlist = nullptr;    // vectors to orthogonalise against, in order
@@ -68,7 +69,9 @@ hipError_t pro_init(ProState &p, const nv_lanczos_opts &o, const Operator &op, c
                     const LzState &st, std::vector<void *> &allocs, hipStream_t s);
 size_t pro_decide_lds(int64_t niter);
 hipError_t pro_reset(ProState &p, const nv_lanczos_opts &o, hipStream_t s);
-hipError_t pro_set_omega(ProState &p, const NvOpDev &dev, const LzState &st, int64_t it, hipStream_t s);
+int64_t pro_check_period(const ProState &p);
+// check != 0: also the exact overlap check of this iteration.
+hipError_t pro_set_omega(ProState &p, const NvOpDev &dev, const LzState &st, int check, hipStream_t s);
 hipError_t pro_partial_orthog(ProState &p, const NvOpDev &dev, const LzState &st, hipStream_t s);
 hipError_t pro_get_omega(const ProState &p, double *omega, int64_t n);
 void pro_stats(const ProState &p, int64_t *events, int64_t *touched);
