@@ -130,6 +130,8 @@ def main():
     print(f"# workload {args.workload}: N={N} niter={args.niter} reorthog={args.reorthog}", file=sys.stderr,
           flush=True)
     with Engine(prob, device=0) as eng:
+        kinfo = {p: eng.hv_kernel(p) for p in ("f32", "f64")}
+        print(f"# H*v kernels: {kinfo}", file=sys.stderr, flush=True)
         dt32, hv32_ms, hv32_n, out32, pro32 = timed(eng, args, "f32", sync)
         print(f"# f32 job {dt32 / args.steps * 1e3:.1f} ms", file=sys.stderr, flush=True)
         res64 = None
@@ -154,7 +156,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": load_traffic(os.path.join(ROOT, "profiles", f"hv_f32_{args.workload}_traffic.txt")),
-                     "kernel": "hv_iter_kernel<3,float,float>", "avg_launch_us": round(hv_avg_s * 1e6, 2),
+                     "kernel": "H*v f32: " + kinfo["f32"], "avg_launch_us": round(hv_avg_s * 1e6, 2),
                      "bytes_per_launch": hv_bytes, "launches": hv32_n},
         "hv_gbps": round(achieved, 1),
         "pro": {"events_per_job": pro32[0], "vectors_touched_per_job": pro32[1]},

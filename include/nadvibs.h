@@ -189,6 +189,10 @@ int32_t nv_profile_enable(nv_ctx *ctx, int32_t on);
 /* Milliseconds spent in the timed H*v launches and their number since the
  * previous call; synchronises. */
 int32_t nv_profile_read(nv_ctx *ctx, double *hv_ms, int64_t *hv_launches);
+/* Which H*v kernel serves precision prec: the operator-specialised one
+ * (compiled at run time, on first use) or the generic one, as text.  The
+ * environment variable NV_HV_KERNEL=generic selects the generic kernel. */
+int32_t nv_hv_kernel_info(nv_ctx *ctx, int32_t prec, char *buf, int64_t len);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnadvibs.so")
 
-SOURCES = ["nv_operator.cpp", "nv_kernels.hip", "nv_pro.hip", "nv_fcstart.hip", "nv_api.hip"]
+SOURCES = ["nv_operator.cpp", "nv_jit.cpp", "nv_kernels.hip", "nv_pro.hip", "nv_fcstart.hip", "nv_api.hip"]
 
 # Packed FP32 instructions are disabled for every device pass.
 HIPFLAGS = [
@@ -43,7 +43,8 @@ def build(verbose=True, jobs=None):
         cmd = [hipcc()] + HIPFLAGS + inc + ["-x", "hip" if src.endswith(".hip") else "c++",
                                           "-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [hipcc(), "-O3", "-std=c++17", "-fPIC"] + inc + ["-c", os.path.join(CSRC, src), "-o", obj]
+            cmd = [hipcc(), "-O3", "-std=c++17", "-fPIC"] + inc + [
+                "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
@@ -60,7 +61,7 @@ def build(verbose=True, jobs=None):
     if failed:
         raise RuntimeError("compilation failed")
     tmp = LIB + ".tmp"
-    link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    link = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lhiprtc"]
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.check_call(link)

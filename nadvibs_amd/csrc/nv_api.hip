@@ -9,6 +9,9 @@
 #include <vector>
 
 #include "nadvibs.h"
+#include <cstdlib>
+
+#include "nv_jit.hpp"
 #include "nv_kernels.hpp"
 #include "nv_operator.hpp"
 #include "nv_pro.hpp"
@@ -39,6 +42,12 @@ struct nv_ctx {
     // [2] re-orthogonalisation with the exact overlap check
     hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t graph = nullptr;
+
+    // operator-specialised H*v kernels per precision (F64, F32)
+    bool jit_on = true;
+    nv::JitKernels jit[2];
+    std::string jit_log;
+    float *hx32 = nullptr;
 
     // H*v timing
     bool prof = false;
@@ -117,6 +126,10 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
         return fail(nullptr, NV_ERR_ARG, err);
     }
     c->device = device;
+    {
+        const char *k = std::getenv("NV_HV_KERNEL");
+        c->jit_on = !(k && std::strcmp(k, "generic") == 0);
+    }
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     const nv::Operator &op = c->op;
@@ -185,6 +198,8 @@ int32_t nv_destroy(nv_ctx *c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     release_lanczos(c);
+    for (auto &k : c->jit) nv::jit_release(k);
+    if (c->hx32) hipFree(c->hx32);
     free_list(c->allocs);
     for (auto &p : c->ev) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
     if (c->d_startvals) hipFree(c->d_startvals);
@@ -216,6 +231,34 @@ int32_t nv_term_counts(const nv_ctx *c, int64_t *out) {
     return NV_OK;
 }
 
+// Compiles the operator-specialised kernels of one precision on first use;
+// when that is not possible the generic kernels serve (both are HIP).
+static const nv::JitKernels *ensure_jit(nv_ctx *c, int32_t prec) {
+    if (!c->jit_on || (prec != NV_PREC_F64 && prec != NV_PREC_F32)) return nullptr;
+    nv::JitKernels &k = c->jit[prec];
+    if (k.ready) return &k;
+    if (k.failed) return nullptr;
+    std::string why;
+    if (!nv::jit_eligible(c->op, prec, why)) {
+        k.failed = true;
+        c->jit_log = why;
+        return nullptr;
+    }
+    if (nv::jit_build(c->op, prec, k, c->jit_log) != hipSuccess) {
+        k.failed = true;
+        return nullptr;
+    }
+    return &k;
+}
+
+static hipError_t launch_hv_iter_any(nv_ctx *c) {
+    const nv::JitKernels *k = ensure_jit(c, c->opts.prec);
+    if (!k) return nv::launch_hv_iter(c->dev, c->opts.prec, c->st, c->stream);
+    hipError_t e = nv::jit_launch_iter(*k, c->dev, c->st, c->stream);
+    if (e != hipSuccess) return e;
+    return nv::launch_finish_sum(c->st.partials, nv::hv_blocks(c->dev), c->st.alpha, c->st.iter, 0, c->stream);
+}
+
 int32_t nv_hv_device(nv_ctx *c, int32_t prec, const double *x, double *y, void *stream) {
     if (!c) return NV_ERR_ARG;
     if (!x || !y) return fail(c, NV_ERR_ARG, "x or y is NULL");
@@ -223,7 +266,18 @@ int32_t nv_hv_device(nv_ctx *c, int32_t prec, const double *x, double *y, void *
         return fail(c, NV_ERR_ARG, "nv_hv supports NV_PREC_F64 and NV_PREC_F32");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     NV_HIP(c, hipSetDevice(c->device));
-    NV_HIP(c, nv::launch_hv_full(c->dev, prec, x, y, s));
+    const nv::JitKernels *k = ensure_jit(c, prec);
+    if (!k) {
+        NV_HIP(c, nv::launch_hv_full(c->dev, prec, x, y, s));
+        return NV_OK;
+    }
+    const void *xin = x;
+    if (prec == NV_PREC_F32) {
+        if (!c->hx32) NV_HIP(c, hipMalloc(&c->hx32, (size_t)c->op.N * sizeof(float)));
+        NV_HIP(c, nv::launch_cast_f32(x, c->hx32, c->op.N, s));
+        xin = c->hx32;
+    }
+    NV_HIP(c, nv::jit_launch_full(*k, c->dev, x, xin, y, s));
     return NV_OK;
 }
 
@@ -338,6 +392,7 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
                             : nv::pro_init(c->pro, c->opts, c->op, c->dev, st, c->lz_allocs, c->stream);
         if (e != hipSuccess) return hip_fail(c, e, "pro_init");
     }
+    ensure_jit(c, o->prec);  // compile before any capture
     // start vector in the slot of iteration 1, beta(0) = |start|
     double *s1 = st.V + (1 % st.nslots) * N;
     int32_t rc = build_start(c, s1);
@@ -380,11 +435,20 @@ int32_t nv_hv_iter(nv_ctx *c, int64_t it) {
         }
         NV_HIP(c, hipEventRecord(c->ev[c->ev_used].first, c->stream));
     }
-    NV_HIP(c, nv::launch_hv_iter(c->dev, c->opts.prec, c->st, c->stream));
+    NV_HIP(c, launch_hv_iter_any(c));
     if (c->prof) {
         NV_HIP(c, hipEventRecord(c->ev[c->ev_used].second, c->stream));
         c->ev_used++;
     }
+    return NV_OK;
+}
+
+int32_t nv_hv_kernel_info(nv_ctx *c, int32_t prec, char *buf, int64_t len) {
+    if (!c || !buf || len < 1) return NV_ERR_ARG;
+    NV_HIP(c, hipSetDevice(c->device));
+    const nv::JitKernels *k = ensure_jit(c, prec);
+    std::string t = k ? "operator-specialised (hiprtc)" : ("generic" + (c->jit_log.empty() ? std::string() : ": " + c->jit_log));
+    std::snprintf(buf, (size_t)len, "%s", t.c_str());
     return NV_OK;
 }
 
@@ -442,7 +506,7 @@ int32_t nv_partial_orthog(nv_ctx *c, int64_t it) {
 static hipError_t enqueue_iteration(nv_ctx *c, int pro, int check) {
     hipError_t e = nv::launch_set_iter(c->d_iter, 0, 1, c->stream);
     if (e == hipSuccess) e = nv::launch_normalize(c->dev, c->opts.prec, c->st, c->stream);
-    if (e == hipSuccess) e = nv::launch_hv_iter(c->dev, c->opts.prec, c->st, c->stream);
+    if (e == hipSuccess) e = launch_hv_iter_any(c);
     if (e == hipSuccess) e = nv::launch_lanczos_step(c->dev, c->opts.prec, c->st, c->stream);
     if (pro) {
         if (e == hipSuccess) e = nv::pro_set_omega(c->pro, c->dev, c->st, check, c->stream);

@@ -39,35 +39,6 @@ __device__ double block_sum(double v) {
     return t;
 }
 
-// Each block contributes `v`; the last block to finish reduces the per-block
-// partials in block order and returns true on every thread of that block
-// with *total set.  The counter is re-armed by the last block.
-__device__ bool last_block_total(double v, double *partials, unsigned *counter, double *total) {
-    __shared__ bool is_last;
-    __shared__ double tot;
-    double b = block_sum(v);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[blockIdx.x], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double s = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-        s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s = block_sum(s);
-    if (threadIdx.x == 0) {
-        tot = s;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    *total = tot;
-    return true;
-}
-
 // ------------------------------------------------------------------------
 // element types
 // ------------------------------------------------------------------------
@@ -287,9 +258,8 @@ __global__ __launch_bounds__(kBlock) void hv_iter_kernel(NvOpDev op, LzState st)
             part += v[s] * (dv[s] + (double)acc[s]);
         }
     }
-    double tot;
-    if (last_block_total(part, st.partials, st.counters + kCntAlpha, &tot))
-        if (tid == 0) st.alpha[it] = tot;
+    const double b = block_sum(part);
+    if (tid == 0) st.partials[blockIdx.x] = b;
 }
 
 // lanczos_step (Source/top.f90:820-842): r = (H q_i - alpha q_i) - beta(i-1) q_{i-1}
@@ -323,9 +293,8 @@ __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzStat
             part += w * w;
         }
     }
-    double tot;
-    if (last_block_total(part, st.partials, st.counters + kCntBeta, &tot))
-        if (tid == 0) st.beta[it] = sqrt(tot);
+    const double bs = block_sum(part);
+    if (tid == 0) st.partials[blockIdx.x] = bs;
 }
 
 // update_lanczos (Source/top.f90:646-670): q_i = r / beta(i-1) in place, and
@@ -355,14 +324,32 @@ __global__ __launch_bounds__(kBlock) void fill_start_kernel(double *v, int64_t D
     }
 }
 
-__global__ __launch_bounds__(kBlock) void norm_kernel(const double *v, int64_t n, double *out, double *partials,
-                                                      unsigned *counter) {
+__global__ __launch_bounds__(kBlock) void norm_kernel(const double *v, int64_t n, double *partials) {
     double p = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         p += v[i] * v[i];
-    double tot;
-    if (last_block_total(p, partials, counter, &tot))
-        if (threadIdx.x == 0) *out = sqrt(tot);
+    const double b = block_sum(p);
+    if (threadIdx.x == 0) partials[blockIdx.x] = b;
+}
+
+// Sums nparts per-block partials in block order (one block; the kernel
+// boundary makes the partials visible) and stores the sum, or its square
+// root, at out[*iter] (iter null: out[0]).
+__global__ __launch_bounds__(kBlock) void finish_sum_kernel(const double *partials, int nparts, double *out,
+                                                           const int64_t *iter, int take_sqrt) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) s += partials[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        const int64_t k = iter ? *iter : 0;
+        out[k] = take_sqrt ? sqrt(s) : s;
+    }
+}
+
+hipError_t launch_finish_sum(const double *partials, int nparts, double *out, const int64_t *iter, int take_sqrt,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kBlock), 0, s, partials, nparts, out, iter, take_sqrt);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------
@@ -443,12 +430,24 @@ hipError_t launch_hv_full(const NvOpDev &op, int prec, const double *x, double *
     NV_DISPATCH_NS(hv_full_ns, op, prec, x, y, s)
 }
 
-hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+static hipError_t hv_iter_any(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     NV_DISPATCH_NS(hv_iter_ns, op, prec, st, s)
 }
 
-hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+static hipError_t step_any(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     NV_DISPATCH_NS(step_ns, op, prec, st, s)
+}
+
+hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    hipError_t e = hv_iter_any(op, prec, st, s);
+    if (e != hipSuccess) return e;
+    return launch_finish_sum(st.partials, hv_blocks(op), st.alpha, st.iter, 0, s);
+}
+
+hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    hipError_t e = step_any(op, prec, st, s);
+    if (e != hipSuccess) return e;
+    return launch_finish_sum(st.partials, hv_blocks(op), st.beta, st.iter, 1, s);
 }
 
 hipError_t launch_normalize(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
@@ -459,6 +458,16 @@ hipError_t launch_normalize(const NvOpDev &op, int prec, const LzState &st, hipS
         hipLaunchKernelGGL((normalize_kernel<float>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N, st);
     else
         hipLaunchKernelGGL((normalize_kernel<__hip_bfloat16>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N, st);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void cast_f32_kernel(const double *x, float *y, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        y[i] = (float)x[i];
+}
+
+hipError_t launch_cast_f32(const double *x, float *y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(cast_f32_kernel, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, x, y, n);
     return hipGetLastError();
 }
 
@@ -474,8 +483,10 @@ hipError_t launch_fill_start(double *v, int64_t D, int ns, const double *vals_de
 
 hipError_t launch_norm_to(const double *v, int64_t n, double *out, double *partials, unsigned *counter,
                           hipStream_t s) {
-    hipLaunchKernelGGL(norm_kernel, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, v, n, out, partials, counter);
-    return hipGetLastError();
+    (void)counter;
+    const int g = grid_stride_blocks(n);
+    hipLaunchKernelGGL(norm_kernel, dim3(g), dim3(kBlock), 0, s, v, n, partials);
+    return launch_finish_sum(partials, g, out, nullptr, 1, s);
 }
 
 // ------------------------------------------------------------------------

@@ -34,7 +34,9 @@ int hv_blocks(const NvOpDev &op);
 hipError_t launch_diag_precompute(const NvOpDev &op, double *dg, hipStream_t s);
 // y = H x, x and y double, full operator (standalone H*v).
 hipError_t launch_hv_full(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s);
-// Converts q_i to the input precision of H*v.
+// y = (float)x.
+hipError_t launch_cast_f32(const double *x, float *y, int64_t n, hipStream_t s);
+// Sets (or increments) the device iteration counter.
 hipError_t launch_set_iter(int64_t *d_iter, int64_t value, int increment, hipStream_t s);
 // update_lanczos: q_i = r_{i-1} / beta(i-1), plus the low-precision copy.
 hipError_t launch_normalize(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
@@ -44,6 +46,9 @@ hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStr
 hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
 // Writes the start vector: zero except vib index 0 of each state, from vals[ns].
 hipError_t launch_fill_start(double *v, int64_t D, int ns, const double *vals_dev, hipStream_t s);
+// out[*iter] (or out[0]) = sum (or sqrt of the sum) of nparts partials.
+hipError_t launch_finish_sum(const double *partials, int nparts, double *out, const int64_t *iter, int take_sqrt,
+                             hipStream_t s);
 // beta[0] = |v| over n doubles.
 hipError_t launch_norm_to(const double *v, int64_t n, double *out, double *partials, unsigned *counter,
                           hipStream_t s);

@@ -393,25 +393,17 @@ __global__ __launch_bounds__(kBlock) void pro_axpy_kernel(int64_t N, ProDev p, L
         w[i] = v;
         part += v * v;
     }
-    // last-block reduction of |w|^2 into beta(it)
-    __shared__ bool last;
     const double b = pro_block_sum(part);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&st.partials[blockIdx.x], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        last = atomicAdd(st.counters + kCntPro, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x == 0) st.partials[blockIdx.x] = b;
+}
+
+// beta(it) = |residual| after the update (only when vectors were listed).
+__global__ __launch_bounds__(kBlock) void pro_beta_kernel(ProDev p, LzState st, int nparts) {
+    if (*p.njl == 0) return;
     double s = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock)
-        s += __hip_atomic_load(&st.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = threadIdx.x; i < nparts; i += kBlock) s += st.partials[i];
     s = pro_block_sum(s);
-    if (threadIdx.x == 0) {
-        st.beta[it] = sqrt(s);
-        __hip_atomic_store(st.counters + kCntPro, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) st.beta[*st.iter] = sqrt(s);
 }
 
 // Exact-check period: the reference's chkorthog; in the mixed modes at most
@@ -531,6 +523,7 @@ hipError_t pro_partial_orthog(ProState &ps, const NvOpDev &dev, const LzState &s
     const int64_t N = dev.N;
     int g = (int)std::min<int64_t>((N + kBlock - 1) / kBlock, 4096);
     hipLaunchKernelGGL(pro_axpy_kernel, dim3(g), dim3(kBlock), 0, s, N, p, st);
+    hipLaunchKernelGGL(pro_beta_kernel, dim3(1), dim3(kBlock), 0, s, p, st, g);
     return hipGetLastError();
 }
 

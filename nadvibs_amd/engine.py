@@ -162,6 +162,12 @@ class Engine:
                                      sg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return ev, it, bj, sg
 
+    def hv_kernel(self, prec="f64"):
+        """Which H*v kernel serves `prec` (compiles it on first use)."""
+        buf = ctypes.create_string_buffer(256)
+        self._check(self.lib.nv_hv_kernel_info(self.h, _lib.PREC[prec], buf, 256))
+        return buf.value.decode()
+
     def profile(self, on=True):
         self._check(self.lib.nv_profile_enable(self.h, 1 if on else 0))
 

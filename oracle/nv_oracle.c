@@ -60,6 +60,10 @@ typedef struct {
     int *bi, *bj;
     double *statew;
     long *ntcom;
+    int neworigin;
+    double *bomega, *dvec, *tmat;
+    long *npirrep, *istate;
+    int nirreps;
 } OOp;
 
 /* ------------------------------------------------------------------ */
@@ -115,6 +119,7 @@ void or_destroy(OOp *op) {
     }
     free(op->terms); free(op->nfunc); free(op->shft); free(op->omega); free(op->concoef);
     free(op->homatel); free(op->bi); free(op->bj); free(op->statew); free(op->ntcom);
+    free(op->bomega); free(op->dvec); free(op->tmat); free(op->npirrep); free(op->istate);
     free(op);
 }
 
@@ -138,6 +143,27 @@ OOp *or_create(const nv_problem *pr) {
     op->N = op->D * op->ns;
     for (int b = 0; b < op->nstblks; ++b) op->concoef[b] = pr->concoef[b];
     for (int s = 0; s < op->ns; ++s) op->statew[s] = pr->statew ? pr->statew[s] : 1.0;
+    op->neworigin = pr->neworigin;
+    op->istate = (long *)calloc(op->M, sizeof(long));
+    if (pr->istate) for (int k = 0; k < op->M; ++k) op->istate[k] = pr->istate[k];
+    op->nirreps = 1;
+    op->npirrep = (long *)malloc(sizeof(long) * (pr->nirreps > 1 ? pr->nirreps : 1));
+    op->npirrep[0] = op->M;
+    if (pr->nirreps > 1 && pr->npirrep) {
+        long tot = 0;
+        for (long g = 0; g < pr->nirreps; ++g) tot += pr->npirrep[g];
+        if (tot == op->M) {
+            op->nirreps = (int)pr->nirreps;
+            for (long g = 0; g < pr->nirreps; ++g) op->npirrep[g] = pr->npirrep[g];
+        }
+    }
+    if (op->neworigin) {
+        op->bomega = (double *)malloc(op->M * sizeof(double));
+        op->dvec = (double *)malloc(op->M * sizeof(double));
+        op->tmat = (double *)malloc((size_t)op->M * op->M * sizeof(double));
+        for (int k = 0; k < op->M; ++k) { op->bomega[k] = pr->bomega[k]; op->dvec[k] = pr->dvec[k]; }
+        for (int k = 0; k < op->M * op->M; ++k) op->tmat[k] = pr->tmat[k];
+    }
     /* shft(i) = prod_{j>i} nfunc(j), Source/top.f90:379-386 */
     for (int i = 0; i < op->M; ++i) {
         op->shft[i] = 1;
@@ -398,10 +424,174 @@ void or_hv(const OOp *op, const double *x, double *y) {
     }
 }
 
-/* default start vector (Source/secondary.f90:727-783) */
+
+/* ------------------------------------------------------------------ */
+/* Franck-Condon start vector (Source/secondary.f90:351-783)            */
+/* ------------------------------------------------------------------ */
+/* Symmetric eigen-decomposition by cyclic Jacobi (the reference uses GIVENS
+ * on the packed A; any accurate decomposition gives the same A^-1, det A). */
+static void o_jacobi(int M, double *A, double *V, double *w) {
+    for (int i = 0; i < M * M; ++i) V[i] = 0.0;
+    for (int i = 0; i < M; ++i) V[i * M + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int i = 0; i < M; ++i) for (int j = i + 1; j < M; ++j) off += A[i * M + j] * A[i * M + j];
+        if (off < 1e-300) break;
+        for (int p = 0; p < M; ++p)
+            for (int q = p + 1; q < M; ++q) {
+                if (A[p * M + q] == 0.0) continue;
+                double th = (A[q * M + q] - A[p * M + p]) / (2 * A[p * M + q]);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < M; ++k) {
+                    double akp = A[k * M + p], akq = A[k * M + q];
+                    A[k * M + p] = c * akp - s * akq; A[k * M + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < M; ++k) {
+                    double apk = A[p * M + k], aqk = A[q * M + k];
+                    A[p * M + k] = c * apk - s * aqk; A[q * M + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < M; ++k) {
+                    double vkp = V[k * M + p], vkq = V[k * M + q];
+                    V[k * M + p] = c * vkp - s * vkq; V[k * M + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < M; ++i) w[i] = A[i * M + i];
+}
+
+static void o_fc_start(const OOp *op, double *q) {
+    const int M = op->M;
+    const long D = op->D;
+    const double *a = op->omega, *b = op->bomega, *d = op->dvec;
+    double *T = (double *)malloc(M * M * sizeof(double)), *W = (double *)malloc(M * M * sizeof(double));
+    double *A = (double *)calloc(M * M, sizeof(double)), *V = (double *)malloc(M * M * sizeof(double));
+    double *lam = (double *)malloc(M * sizeof(double)), *Ai = (double *)calloc(M * M, sizeof(double));
+    double *A1 = (double *)calloc(M * M, sizeof(double)), *A2 = (double *)calloc(M * M, sizeof(double));
+    double *A3 = (double *)calloc(M * M, sizeof(double));
+    double *A2d = (double *)calloc(M, sizeof(double)), *A3d = (double *)calloc(M, sizeof(double));
+    int *irrep = (int *)malloc(M * sizeof(int));
+    for (int i = 0; i < M; ++i) for (int j = 0; j < M; ++j) T[j * M + i] = op->tmat[i * M + j];
+    /* |det T| with partial pivoting (gauss_elim, :311-348) */
+    memcpy(W, T, M * M * sizeof(double));
+    double detT = 1.0;
+    for (int i = 0; i < M; ++i) {
+        int p = i;
+        for (int j = i + 1; j < M; ++j) if (fabs(W[j * M + i]) > fabs(W[p * M + i])) p = j;
+        if (p != i) for (int k = 0; k < M; ++k) { double t = W[i * M + k]; W[i * M + k] = W[p * M + k]; W[p * M + k] = t; }
+        for (int j = i + 1; j < M; ++j) {
+            if (W[j * M + i] == 0.0) continue;
+            double v = -W[j * M + i] / W[i * M + i];
+            for (int k = 0; k < M; ++k) W[j * M + k] += v * W[i * M + k];
+        }
+        detT *= fabs(W[i * M + i]);
+    }
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < M; ++k) s += 0.5 * b[k] * T[k * M + i] * T[k * M + j];
+            if (i == j) s += 0.5 * a[i];
+            A[i * M + j] = s;
+        }
+    o_jacobi(M, A, V, lam);
+    double detA = 1.0;
+    for (int i = 0; i < M; ++i) detA *= fabs(lam[i]);
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < M; ++k) s += V[i * M + k] * V[j * M + k] / lam[k];
+            Ai[i * M + j] = s;
+        }
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) {
+            A1[i * M + j] = Ai[i * M + j] * sqrt(a[i]) * sqrt(a[j]);
+            double s2 = 0.0, s3 = 0.0;
+            for (int k = 0; k < M; ++k) {
+                s2 += Ai[i * M + k] * T[j * M + k];
+                for (int l = 0; l < M; ++l) s3 += T[i * M + k] * Ai[k * M + l] * T[j * M + l];
+            }
+            A2[i * M + j] = s2 * sqrt(a[i]) * sqrt(b[j]);
+            A3[i * M + j] = s3 * sqrt(b[i]) * sqrt(b[j]);
+        }
+    for (int i = 0; i < M; ++i) {
+        A1[i * M + i] -= 1.0;
+        for (int j = 0; j < M; ++j) {
+            double d1 = A3[j * M + i] - (i == j ? 2.0 : 0.0);
+            A2d[i] += A2[i * M + j] * d[j] * sqrt(b[j]);
+            A3d[i] += d1 * d[j] * sqrt(b[j]);
+        }
+    }
+    { int m0 = 0; for (int g = 0; g < op->nirreps; ++g) for (long k = 0; k < op->npirrep[g]; ++k) irrep[m0++] = g; }
+    /* <0|n>: levels of total quanta, every function of level L pushes to its
+     * neighbours n + e_j (Source/secondary.f90:487-570), in index order */
+    double *C = (double *)calloc(D, sizeof(double));
+    int *lev = (int *)malloc(D * sizeof(int));
+    int nlev = 0;
+    for (int k = 0; k < M; ++k) nlev += (int)op->nfunc[k] - 1;
+    for (long r = 0; r < D; ++r) {
+        int L = 0;
+        for (int k = 0; k < M; ++k) L += (int)((r / op->shft[k]) % op->nfunc[k]);
+        lev[r] = L;
+    }
+    C[0] = 1.0;
+    for (int L = 0; L < nlev; ++L)
+        for (long r = 0; r < D; ++r) {
+            if (lev[r] != L) continue;
+            for (int j = 0; j < M; ++j) {
+                long nj = (r / op->shft[j]) % op->nfunc[j];
+                if (nj + 1 >= op->nfunc[j]) continue;
+                double s = 0.0;
+                for (int k = 0; k < M; ++k) {
+                    if (irrep[k] != irrep[j]) continue;
+                    long nk = (r / op->shft[k]) % op->nfunc[k];
+                    if (nk > 0) s += A1[j * M + k] * nk * C[r - op->shft[k]];
+                }
+                C[r + op->shft[j]] = 2.0 * s - A2d[j] * C[r];
+            }
+        }
+    /* one excited mode: recursion in m (:602-689) */
+    int im = 0;
+    while (im < M - 1 && op->istate[im] <= 0) ++im;
+    int ninit = (int)op->istate[im];
+    double *Cp = (double *)calloc(D, sizeof(double)), *Cn = (double *)calloc(D, sizeof(double));
+    for (int i = 1; i <= ninit; ++i) {
+        for (long r = 0; r < D; ++r) {
+            double v = 2.0 * Cp[r] * (i - 1) * (A3[im * M + im] - 1.0) - A3d[im] * C[r];
+            for (int k = 0; k < M; ++k) {
+                long nk = (r / op->shft[k]) % op->nfunc[k];
+                if (nk > 0) v += 2.0 * A2[k * M + im] * nk * C[r - op->shft[k]];
+            }
+            Cn[r] = v;
+        }
+        double *t = Cp; Cp = C; C = Cn; Cn = t;
+    }
+    /* G and the scale factors (:694-725) */
+    double e1 = 0.0, pab = 1.0;
+    for (int i = 0; i < M; ++i) e1 -= 0.5 * b[i] * d[i] * d[i];
+    for (int i = 0; i < M; ++i) for (int j = 0; j < M; ++j) e1 += 0.25 * sqrt(b[i]) * d[i] * A3[i * M + j] * d[j] * sqrt(b[j]);
+    for (int i = 0; i < M; ++i) pab *= b[i] * a[i];
+    const double G = sqrt(sqrt(pab) * detT / detA) * exp(e1);
+    for (long r = 0; r < D; ++r) {
+        double dd = 1.0;
+        for (int k = 0; k < M; ++k) {
+            long n = (r / op->shft[k]) % op->nfunc[k];
+            double f = 1.0; for (long t = 2; t <= n; ++t) f *= t;
+            double g = 1.0; for (long t = 2; t <= op->istate[k]; ++t) g *= t;
+            dd *= pow(2.0, (double)n) * f * pow(2.0, (double)op->istate[k]) * g;
+        }
+        C[r] = G * C[r] / sqrt(dd);
+    }
+    for (int s = 0; s < op->ns; ++s) memcpy(q + (size_t)s * D, C, D * sizeof(double));
+    free(T); free(W); free(A); free(V); free(lam); free(Ai); free(A1); free(A2); free(A3); free(A2d); free(A3d);
+    free(irrep); free(C); free(lev); free(Cp); free(Cn);
+}
+
+/* start vector: FC overlaps or |0> in every state, then weights and
+ * normalisation (Source/secondary.f90:727-783) */
 void or_start_vector(const OOp *op, double *q) {
     memset(q, 0, (size_t)op->N * sizeof(double));
-    for (int s = 0; s < op->ns; ++s) q[s * op->D] = 1.0;
+    if (op->neworigin) o_fc_start(op, q);
+    else for (int s = 0; s < op->ns; ++s) q[s * op->D] = 1.0;
     double d1 = 0.0;
     for (long i = 0; i < op->N; ++i) d1 += q[i] * q[i];
     d1 = sqrt(d1 / op->ns);

@@ -75,9 +75,19 @@ class Oracle:
                       np.ascontiguousarray(p.statew if p.statew is not None else np.ones(p.nstates),
                                            dtype=np.float64)]
         nf, ao, cc, pt, sw = self._keep
+        neworigin = int(bool(getattr(p, "shiftref", 0)))
+        extra = {}
+        if neworigin:
+            bo, dv, tm = (np.ascontiguousarray(v, dtype=np.float64) for v in (p.bomega, p.dvec, p.tmat))
+            self._keep += [bo, dv, tm]
+            extra.update(bomega=_d(bo), dvec=_d(dv), tmat=_d(tm))
+        npirr = np.ascontiguousarray(p.npirrep or [p.nmodes], dtype=np.int64)
+        ist = np.ascontiguousarray(p.initstate or [0] * p.nmodes, dtype=np.int64)
+        self._keep += [npirr, ist]
         desc = _Problem(nstates=p.nstates, nmodes=p.nmodes, ordr=p.ordr, nfunc=nf.ctypes.data_as(_i64p),
                         aomega=_d(ao), concoef=_d(cc), potterms=_d(pt), ztoler=p.ztoler, quirks=p.quirks,
-                        neworigin=0, statew=_d(sw), nirreps=1)
+                        neworigin=neworigin, statew=_d(sw), nirreps=len(npirr), npirrep=npirr.ctypes.data_as(_i64p),
+                        istate=ist.ctypes.data_as(_i64p), **extra)
         self.h = lib().or_create(ctypes.byref(desc))
         if not self.h:
             raise ValueError("oracle rejected the problem")

@@ -6,6 +6,8 @@ first iterations 1e-11 relative (rounding differences grow with the
 iteration count through the usual loss of orthogonality); eigenvalues of
 the same tridiagonal 1e-13 relative.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -34,12 +36,30 @@ def _prob(case, seed=11):
                         quirks=quirks, offdiag_even=offe)
 
 
+@pytest.fixture(params=["specialised", "generic"])
+def hv_kernel(request, monkeypatch):
+    """Both H*v kernels: the operator-specialised one (compiled at run time)
+    and the generic one (NV_HV_KERNEL=generic)."""
+    if request.param == "generic":
+        monkeypatch.setenv("NV_HV_KERNEL", "generic")
+    else:
+        monkeypatch.delenv("NV_HV_KERNEL", raising=False)
+    return request.param
+
+
+def _check_kernel(eng, kind, prec):
+    info = eng.hv_kernel(prec)
+    print(f"H*v kernel ({prec}): {info}", flush=True)
+    assert info.startswith("operator-specialised" if kind == "specialised" else "generic"), info
+
+
 @pytest.mark.parametrize("case", CASES, ids=[f"c{i}" for i in range(len(CASES))])
-def test_hv_f64_matches_oracle(case):
+def test_hv_f64_matches_oracle(case, hv_kernel):
     p = _prob(case)
     print(f"case {case} n={p.n}", flush=True)
     orc = Oracle(p)
     with Engine(p) as eng:
+        _check_kernel(eng, hv_kernel, "f64")
         np.testing.assert_array_equal(eng.term_counts(), orc.term_counts())
         np.testing.assert_array_equal(eng.term_summary(), orc.term_summary())
         for seed in range(3):
@@ -51,10 +71,11 @@ def test_hv_f64_matches_oracle(case):
 
 
 @pytest.mark.parametrize("case", CASES[:4], ids=[f"c{i}" for i in range(4)])
-def test_hv_f32_matches_oracle(case):
+def test_hv_f32_matches_oracle(case, hv_kernel):
     p = _prob(case)
     orc = Oracle(p)
     with Engine(p) as eng:
+        _check_kernel(eng, hv_kernel, "f32")
         x = np.random.default_rng(5).standard_normal(p.n)
         y = eng.hv(x, "f32")
         ref = orc.hv(x)
@@ -75,11 +96,12 @@ def test_hv_cfg1_and_start_vector():
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-11), ("f32", 5e-6)])
-def test_lanczos_coefficients_cfg1(prec, tol):
+def test_lanczos_coefficients_cfg1(prec, tol, hv_kernel):
     p = workload("cfg1")
     orc = Oracle(p)
     ref = orc.lanczos(40)
     with Engine(p) as eng:
+        _check_kernel(eng, hv_kernel, prec)
         eng.lanczos_init(niter=40, prec=prec)
         for i in range(1, 41):  # the reference's loop, Source/main.f90:63-71
             eng.update_lanczos(i)
