@@ -40,7 +40,7 @@ std::string jit_source(const Operator &op, int prec) {
     const int ns = (int)op.ns;
     const int64_t D = op.D;
     std::ostringstream s;
-    s << "#include <hip/hip_runtime.h>\n";
+    // hiprtc provides the HIP device runtime itself (no includes)
     s << "typedef " << T << " T;\n";
     s << "#define NV_D " << D << "LL\n#define NV_N " << op.N << "LL\n";
     // factor tables of the codes used by the steps
