@@ -101,6 +101,30 @@ def load_traffic(path):
     return None
 
 
+def pmc_probe(args):
+    """Kernels for the HBM-traffic counters (run under rocprofv3 --pmc):
+    a device copy of known size (the calibration) and repeated H*v launches
+    of the mixed job's first iteration."""
+    import torch
+    from nadvibs_amd.engine import Engine
+    from nadvibs_amd.synth import workload
+    prob = workload(args.workload)
+    src = torch.empty(64 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    with Engine(prob, device=0) as eng:
+        eng.lanczos_init(niter=4, reorthog=0, prec="f32")
+        eng.update_lanczos(1)
+        for _ in range(args.steps):
+            eng.Hv(1)
+        eng.lanczos_step(1)
+        eng.coefficients(n=1)
+    print(json.dumps({"copy_bytes": 64 * 1024 * 1024, "hv_launches": args.steps, "n": prob.n,
+                      "hv_kernel": "nv_hv_iter"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,11 +135,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only the calibration copy and H*v launches, for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the engine runs on a single "
               f"MI355X; multi-GPU sharding is a design only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.pmc_probe:
+        pmc_probe(args)
+        return
 
     import torch
     from nadvibs_amd.engine import Engine

@@ -168,6 +168,12 @@ int32_t nv_ritz(nv_ctx *ctx, int64_t n, const double *alpha,
                 const double *beta, double *eval, double *inten, double *bji,
                 int32_t *sign);
 
+/* ---- spectrum broadening (convolve.py:28-48, the Gaussian aperture) ----- */
+/* env[g] = sum_j y[j] exp(-((grid[g] - x[j]) / sigma)^2 / 2), divided by its
+ * maximum over the grid (sigma is a standard deviation, not a FWHM). */
+int32_t nv_convolve(nv_ctx *ctx, int64_t n, const double *x, const double *y,
+                    int64_t ng, const double *grid, double sigma, double *env);
+
 /* ---- restart files (make_restart / load_restartinfo,
  *      Source/top.f90:1007-1049, Source/secondary.f90:872-892) -------------- */
 /* Writes q1, q2, q3 of the last iteration as nstates records each of dimen

@@ -67,6 +67,8 @@ SIGNATURES = [
     ("nv_pro_stats", ctypes.c_int32, [ctypes.c_void_p, _i64p, _i64p]),
     ("nv_get_lanczos_vector", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp]),
     ("nv_ritz", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _i32p]),
+    ("nv_convolve", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp,
+                                     ctypes.c_double, _dp]),
     ("nv_save_vectors", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p]),
     ("nv_load_vectors", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64, _dp, _dp, _dp]),
     ("nv_profile_enable", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),

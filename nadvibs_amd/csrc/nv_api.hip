@@ -672,6 +672,29 @@ int32_t nv_ritz(nv_ctx *c, int64_t n, const double *alpha, const double *beta, d
     return NV_OK;
 }
 
+// ---- Gaussian broadening ----------------------------------------------------
+int32_t nv_convolve(nv_ctx *c, int64_t n, const double *x, const double *y, int64_t ng, const double *grid,
+                    double sigma, double *env) {
+    if (!c) return NV_ERR_ARG;
+    if (n < 1 || ng < 1 || !x || !y || !grid || !env || !(sigma > 0))
+        return fail(c, NV_ERR_ARG, "bad convolve arguments");
+    NV_HIP(c, hipSetDevice(c->device));
+    const int nb = (int)((ng + nv::kBlock - 1) / nv::kBlock);
+    double *d = nullptr;
+    const size_t bytes = (size_t)(2 * n + 2 * ng + nb) * sizeof(double);
+    NV_HIP(c, hipMalloc(&d, bytes));
+    double *dx = d, *dy = d + n, *dg = d + 2 * n, *de = d + 2 * n + ng, *dp = d + 2 * n + 2 * ng;
+    hipError_t e = hipMemcpyAsync(dx, x, n * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dy, y, n * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dg, grid, ng * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = nv::launch_convolve(dx, dy, n, dg, ng, sigma, de, dp, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(env, de, ng * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (e != hipSuccess) return hip_fail(c, e, "convolve");
+    return NV_OK;
+}
+
 // ---- restart vectors ------------------------------------------------------
 int32_t nv_save_vectors(nv_ctx *c, const char *path) {
     if (!c || !path) return NV_ERR_ARG;

@@ -490,6 +490,66 @@ hipError_t launch_norm_to(const double *v, int64_t n, double *out, double *parti
 }
 
 // ------------------------------------------------------------------------
+// Gaussian broadening of a stick spectrum (convolve.py:28-48):
+// env[g] = sum_j y_j exp(-((grid_g - x_j)/sigma)^2 / 2), then env / max(env).
+// One thread per grid point; the sticks are staged through LDS in tiles.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void convolve_kernel(const double *x, const double *y, int64_t n,
+                                                          const double *grid, int64_t ng, double sigma, double *env,
+                                                          double *part) {
+    __shared__ double sx[kBlock], sy[kBlock];
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double gx = g < ng ? grid[g] : 0.0;
+    const double inv = 1.0 / sigma;
+    double acc = 0.0;
+    for (int64_t j0 = 0; j0 < n; j0 += kBlock) {
+        __syncthreads();
+        if (j0 + threadIdx.x < n) {
+            sx[threadIdx.x] = x[j0 + threadIdx.x];
+            sy[threadIdx.x] = y[j0 + threadIdx.x];
+        }
+        __syncthreads();
+        const int m = (int)(n - j0 < kBlock ? n - j0 : kBlock);
+        for (int j = 0; j < m; ++j) {
+            const double t = (gx - sx[j]) * inv;
+            acc += sy[j] * exp(-0.5 * t * t);
+        }
+    }
+    if (g < ng) env[g] = acc;
+    // block maximum for the normalisation
+    double mx = g < ng ? acc : -INFINITY;
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    __shared__ double red[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m2 = red[0];
+        for (int w = 1; w < kBlock / 64; ++w) m2 = fmax(m2, red[w]);
+        part[blockIdx.x] = m2;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void convolve_norm_kernel(double *env, int64_t ng, const double *part, int nb) {
+    __shared__ double mx;
+    if (threadIdx.x == 0) {
+        double m = part[0];
+        for (int b = 1; b < nb; ++b) m = fmax(m, part[b]);
+        mx = m;
+    }
+    __syncthreads();
+    for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < ng; g += (int64_t)gridDim.x * kBlock)
+        env[g] /= mx;
+}
+
+hipError_t launch_convolve(const double *x, const double *y, int64_t n, const double *grid, int64_t ng,
+                           double sigma, double *env, double *part, hipStream_t s) {
+    const int nb = (int)((ng + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(convolve_kernel, dim3(nb), dim3(kBlock), 0, s, x, y, n, grid, ng, sigma, env, part);
+    hipLaunchKernelGGL(convolve_norm_kernel, dim3(nb), dim3(kBlock), 0, s, env, ng, part, nb);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
 // Tridiagonal eigensolve (restates GIVENS, Source/yarkony.f90:91-543, for a
 // tridiagonal input: the Householder stage and the back transformation are
 // identities there).  The matrix arrives divided by its Frobenius norm.

@@ -53,6 +53,11 @@ hipError_t launch_finish_sum(const double *partials, int nparts, double *out, co
 hipError_t launch_norm_to(const double *v, int64_t n, double *out, double *partials, unsigned *counter,
                           hipStream_t s);
 
+// Normalised Gaussian envelope of sticks (x, y) on a grid; part holds
+// ceil(ng/256) doubles.
+hipError_t launch_convolve(const double *x, const double *y, int64_t n, const double *grid, int64_t ng,
+                           double sigma, double *env, double *part, hipStream_t s);
+
 // ---- tridiagonal eigensolve ----
 struct TriWork {
     double *d = nullptr;     // [n] normalised diagonal

@@ -168,6 +168,14 @@ class Engine:
         self._check(self.lib.nv_hv_kernel_info(self.h, _lib.PREC[prec], buf, 256))
         return buf.value.decode()
 
+    def convolve(self, x, y, grid, sigma):
+        """Normalised Gaussian envelope of sticks (x, y) on `grid` (convolve.py:28-48)."""
+        x, y, grid = (np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, grid))
+        env = np.empty_like(grid)
+        self._check(self.lib.nv_convolve(self.h, x.size, dptr(x), dptr(y), grid.size, dptr(grid), float(sigma),
+                                         dptr(env)))
+        return env
+
     def profile(self, on=True):
         self._check(self.lib.nv_profile_enable(self.h, 1 if on else 0))
 
