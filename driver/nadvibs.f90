@@ -145,6 +145,11 @@ program nadvibs_main
     if (trim(env) == 'f32') prec = NV_PREC_F32
     opts%niter = niter; opts%reorthog = reorthog; opts%prec = prec; opts%chkorthog = chkorthog
     opts%seed = 20240101_c_int64_t; opts%noise_scale = 1d0; opts%pro_eps = 0d0
+    ! savevecs (Source/secondary.f90:863): the root analysis reads every Lanczos vector
+    opts%savevecs = 0
+    if (idroots > 0 .or. soroots > 0) opts%savevecs = 1
+    if (restart /= 0 .and. opts%savevecs /= 0) &
+        stop 'nadvibs: idroots/soroots need every Lanczos vector and cannot follow a restart'
     call nv_check(nv_lanczos_init(ctx, opts), ctx, 'nv_lanczos_init')
     if (restart /= 0) then
         it = nload
@@ -185,6 +190,7 @@ program nadvibs_main
 
     call compute_eigenvalues()
     close(13)
+    if (idroots > 0 .or. soroots > 0) call identify_roots()
     call nv_check(nv_destroy(ctx), c_null_ptr, 'nv_destroy')
     write(*, *) 'NadVibs terminated.'
 
@@ -239,5 +245,87 @@ contains
         write(13, '(a)') ''
         write(13, '(a)') '   All requested eigenvalues computed.'
     end subroutine compute_eigenvalues
+
+    ! Ritz vectors of the lowest max(idroots, soroots) roots, rootinfo.dat and
+    ! sodata.in (identify_roots, Source/top.f90:1122-1283).
+    subroutine identify_roots()
+        integer :: nroots, kk, q, j, k, l, cnt, m
+        integer(c_int64_t) :: lab
+        real(c_double), allocatable :: ev(:), inten(:), mag2(:, :), pct(:, :), ovl(:, :, :, :)
+        integer(c_int32_t), allocatable :: sgn(:), st(:, :), ph(:, :)
+        integer(c_int64_t), allocatable :: idx(:, :)
+        integer, allocatable :: barray(:)
+        real(8) :: dp(100)
+        character(len=*), parameter :: fmt3 = &
+            '(i5,1x,i5,2x,es14.6,1x,es14.6,1x,es14.6,/i5,1x,i5,2x,es14.6,1x,es14.6,1x,es14.6)'
+        nroots = max(idroots, soroots)
+        kk = max(idroots, 1)
+        allocate(ev(nroots), inten(nroots), sgn(nroots), mag2(kk, nroots), idx(kk, nroots), st(kk, nroots), &
+                 ph(kk, nroots), pct(nstates, nroots), ovl(nstates, nstates, nroots, nroots), barray(nmodes))
+        call nv_check(nv_identify_roots(ctx, int(niter, c_int64_t), alpha, beta, int(nroots, c_int64_t), &
+                      int(idroots, c_int64_t), ev, inten, sgn, idx, st, ph, mag2, pct, ovl, c_null_ptr), &
+                      ctx, 'nv_identify_roots')
+        open(33, file='rootinfo.dat', status='replace')
+        write(33, '(a,i4,a)') ' Summary for computing ', nroots, ' eigenvectors corresponding to the lowest eigenvalues'
+        do q = 1, nroots
+            write(33, '(a,i4,a)') ' Eigenvector ', q, ' has been computed...'
+            write(33, '(a)') ''
+            write(33, '(a,i4,a)') ' ************ ROOT ', q, ' *************'
+            write(33, '(a,f9.2,a,f9.5,a,f6.4/)') ' Energy: ', ev(q) * AU2WAVE, ' cm-1, ', ev(q) * AU2EV, &
+                ' eV   Intensity: ', inten(q)
+            do j = 1, idroots
+                write(33, '(a,i2,a,i12,a,i2,a,f8.6)') ' State vector: ', st(j, q), ' | Index: ', idx(j, q), &
+                    ' Phase: ', ph(j, q), ' | Magnitude^2: ', mag2(j, q)
+                ! get_barray (Source/secondary.f90:812-824)
+                lab = max(idx(j, q) - 1, 0_c_int64_t)
+                do m = nmodes, 1, -1
+                    barray(m) = int(mod(lab, int(basis(m), c_int64_t)))
+                    lab = lab / basis(m)
+                end do
+                write(33, '(a,25(i3))') ' v(i): ', barray
+            end do
+            write(33, '(a)') ' '
+            if (nstates == 2) write(33, '(a,f6.2,a,f6.2,a)') ' Percent State (1,2): (', pct(1, q), ' %, ', &
+                pct(2, q), ' %)'
+            if (nstates == 3) write(33, '(a,f6.2,a,f6.2,a,f6.2,a)') ' Percent State (1,2,3): (', pct(1, q), ' %, ', &
+                pct(2, q), ' %, ', pct(3, q), ' %)'
+            if (nstates == 4) write(33, '(a,f6.2,a,f6.2,a,f6.2,a,f6.2,a)') ' Percent State (1,2,3,4): (', &
+                pct(1, q), ' %, ', pct(2, q), ' %, ', pct(3, q), ' %, ', pct(4, q), ' %)'
+            if (nstates == 5) write(33, '(a,f6.2,a,f6.2,a,f6.2,a,f6.2,a,f6.2,a)') ' Percent State (1,2,3,4,5): (', &
+                pct(1, q), ' %, ', pct(2, q), ' %, ', pct(3, q), ' %, ', pct(4, q), ' %, ', pct(5, q), ' %)'
+            write(33, '(a)') ' --------------------------------------------------------------'
+        end do
+        write(33, '(a)') ' '
+        write(33, '(a)') '------------ FINISHED EXECUTION ------------'
+        close(33)
+        if (soroots <= 0) return
+        open(34, file='sodata.in', status='replace')
+        write(34, '(i5)') soroots
+        write(34, '(a)') ' Root   Energy (eV)      Sqrt(Int.)*sign'
+        do q = 1, soroots
+            write(34, '(i5,2x,es15.8,2x,es15.8)') q, ev(q) * AU2EV, sqrt(inten(q)) * sgn(q)
+        end do
+        if (nstates == 2) write(34, '(a)') ' R(i)  R(j)   [S1(i)S2(j)]   [S2(i)S1(j)]'
+        if (nstates == 3) then
+            write(34, '(a)') ' R(i)  R(j)   [S1(i)S2(j)]   [S1(i)S3(j)]  [S2(i)S1(j)]'
+            write(34, '(a)') ' R(i)  R(j)   [S2(i)S3(j)]   [S3(i)S1(j)]  [S3(i)S2(j)]'
+        end if
+        do q = 1, soroots
+            do j = 1, q - 1
+                cnt = 0
+                do k = 1, nstates
+                    do l = 1, nstates
+                        if (k /= l) then
+                            cnt = cnt + 1
+                            dp(cnt) = ovl(l, k, j, q)
+                        end if
+                    end do
+                end do
+                if (nstates == 2) write(34, '(i5,1x,i5,2x,es14.6,1x,es14.6)') q, j, dp(1), dp(2)
+                if (nstates == 3) write(34, fmt3) q, j, dp(1), dp(2), dp(3), q, j, dp(4), dp(5), dp(6)
+            end do
+        end do
+        close(34)
+    end subroutine identify_roots
 
 end program nadvibs_main

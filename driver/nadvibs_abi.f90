@@ -23,6 +23,7 @@ module nadvibs_abi
         integer(c_int64_t) :: chkorthog
         integer(c_int64_t) :: seed
         real(c_double) :: noise_scale, pro_eps
+        integer(c_int32_t) :: savevecs
     end type nv_lanczos_opts
 
     interface
@@ -96,6 +97,17 @@ module nadvibs_abi
             real(c_double), intent(in) :: alpha(*), beta(*)
             real(c_double), intent(out) :: eval(*), inten(*), bji(*)
             integer(c_int32_t), intent(out) :: sgn(*)
+        end function
+        integer(c_int32_t) function nv_identify_roots(ctx, n, alpha, beta, nroots, idroots, eval, inten, sgn, &
+                top_index, top_state, top_phase, top_mag2, percent, overlap, vectors) bind(C, name="nv_identify_roots")
+            import :: c_int32_t, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: n, nroots, idroots
+            real(c_double), intent(in) :: alpha(*), beta(*)
+            real(c_double), intent(out) :: eval(*), inten(*), top_mag2(*), percent(*), overlap(*)
+            integer(c_int32_t), intent(out) :: sgn(*), top_state(*), top_phase(*)
+            integer(c_int64_t), intent(out) :: top_index(*)
+            type(c_ptr), value :: vectors
         end function
         integer(c_int32_t) function nv_save_vectors(ctx, path) bind(C, name="nv_save_vectors")
             import :: c_int32_t, c_ptr, c_char

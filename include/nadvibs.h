@@ -130,6 +130,11 @@ typedef struct nv_lanczos_opts {
     double pro_eps;           /* unit roundoff assumed by the omega recurrence and
                                  its thresholds sqrt(eps), eps^(3/4); 0 = 2^-53,
                                  the reference's determine_epsilon */
+    int32_t savevecs;         /* != 0: keep every Lanczos vector in device memory
+                                 (the reference's savevecs, set there by
+                                 reorthog > 0, idroots > 0 or soroots > 0,
+                                 Source/secondary.f90:863); needed by
+                                 nv_identify_roots.  reorthog > 0 implies it. */
 } nv_lanczos_opts;
 
 /* Allocates the Krylov basis for opts->niter iterations, builds the start
@@ -167,6 +172,33 @@ int32_t nv_get_lanczos_vector(nv_ctx *ctx, int64_t j, double *q);
 int32_t nv_ritz(nv_ctx *ctx, int64_t n, const double *alpha,
                 const double *beta, double *eval, double *inten, double *bji,
                 int32_t *sign);
+
+/* ---- Ritz vectors and root analysis (identify_roots,
+ *      Source/top.f90:1122-1283) ------------------------------------------- */
+/* For the lowest nroots roots of T(alpha, beta) (n <= last iteration; the
+ * Krylov basis must be kept): Ritz vectors Y_i = sum_k v(k,i) q_k, formed on
+ * the device from the archived basis.  Outputs (any may be NULL except as
+ * noted):
+ *   eval[nroots]       eigenvalues (hartree, absolute);
+ *   inten[nroots]      v(1,i)^2 divided by its maximum over the nroots roots;
+ *   sign[nroots]       sign of v(1,i);
+ *   top_*[nroots*idroots] (required when idroots > 0): the idroots largest
+ *                      |Y_i|^2 elements in decreasing order (ties by
+ *                      increasing Fortran index), as top_index = 1-based
+ *                      vibrational index (get_barray decodes it), top_state =
+ *                      1-based state, top_phase = sign of the element,
+ *                      top_mag2 = its square;
+ *   percent[nroots*nstates]  100 |Y_i[:,k].Y_i[:,k]| / sum_k (state weights);
+ *   overlap[((i*nroots + j)*nstates + k)*nstates + l] = Y_i[:,k] . Y_j[:,l]
+ *                      (the spin-orbit overlaps of sodata.in for i > j, k != l);
+ *   vectors[nroots*N]  the Ritz vectors themselves.
+ * nroots * nstates <= 4096.  Synchronises. */
+int32_t nv_identify_roots(nv_ctx *ctx, int64_t n, const double *alpha,
+                          const double *beta, int64_t nroots, int64_t idroots,
+                          double *eval, double *inten, int32_t *sign,
+                          int64_t *top_index, int32_t *top_state,
+                          int32_t *top_phase, double *top_mag2,
+                          double *percent, double *overlap, double *vectors);
 
 /* ---- spectrum broadening (convolve.py:28-48, the Gaussian aperture) ----- */
 /* env[g] = sum_j y[j] exp(-((grid[g] - x[j]) / sigma)^2 / 2), divided by its

@@ -354,10 +354,11 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     NV_HIP(c, hipSetDevice(c->device));
     NV_HIP(c, hipStreamSynchronize(c->stream));
     const int64_t N = c->op.N;
-    const bool keep = o->reorthog > 0;
+    const bool keep = o->reorthog > 0 || o->savevecs != 0;
     // buffers of an earlier run with the same shape are reused
     const bool same = c->lz_ready && c->opts.niter == o->niter && c->opts.prec == o->prec &&
-                      (c->opts.reorthog > 0) == keep && (c->opts.reorthog > 1) == (o->reorthog > 1);
+                      (c->opts.reorthog > 0 || c->opts.savevecs != 0) == keep &&
+                      (c->opts.reorthog > 0) == (o->reorthog > 0) && (c->opts.reorthog > 1) == (o->reorthog > 1);
     nv::LzState &st = c->st;
     // captured iteration graphs hold the options by value
     if (!same || std::memcmp(&c->opts, o, sizeof(*o)) != 0) release_graphs(c);
@@ -387,7 +388,7 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     NV_HIP(c, hipMemsetAsync(st.alpha, 0, (o->niter + 3) * sizeof(double), c->stream));
     NV_HIP(c, hipMemsetAsync(st.beta, 0, (o->niter + 3) * sizeof(double), c->stream));
     NV_HIP(c, hipMemsetAsync(st.V, 0, (size_t)std::min<int64_t>(st.nslots, 3) * N * sizeof(double), c->stream));
-    if (keep) {
+    if (o->reorthog > 0) {
         hipError_t e = same ? nv::pro_reset(c->pro, c->opts, c->stream)
                             : nv::pro_init(c->pro, c->opts, c->op, c->dev, st, c->lz_allocs, c->stream);
         if (e != hipSuccess) return hip_fail(c, e, "pro_init");
@@ -605,63 +606,92 @@ int32_t nv_get_lanczos_vector(nv_ctx *c, int64_t j, double *q) {
 }
 
 // ---- tridiagonal eigensolve ---------------------------------------------
-int32_t nv_ritz(nv_ctx *c, int64_t n, const double *alpha, const double *beta, double *eval, double *inten,
-                double *bji, int32_t *sign) {
-    if (!c) return NV_ERR_ARG;
-    if (n < 1 || !alpha || !beta || !eval) return fail(c, NV_ERR_ARG, "bad tridiagonal arguments");
+// Eigenvalues (ascending) and eigenvectors of the tridiagonal T(alpha,
+// beta(1:n-1)) on the device.  *vec_dev (n*n doubles, vec[j*n + i] =
+// component j of root i) belongs to the caller, who frees it with hipFree.
+static int32_t tridiag_solve(nv_ctx *c, int64_t n, const double *alpha, const double *beta,
+                             std::vector<double> &root, double **vec_dev) {
+    *vec_dev = nullptr;
     for (int64_t i = 0; i < n; ++i)
         if (!std::isfinite(alpha[i]) || !std::isfinite(beta[i + 1]))
             return fail(c, NV_ERR_ARG, "alpha/beta hold a non-finite value at iteration " + std::to_string(i + 1));
     NV_HIP(c, hipSetDevice(c->device));
-    std::vector<double> root(n), v1(n), vn(n);
+    root.assign(n, 0.0);
     // Frobenius scaling in the order GIVENS accumulates it (Source/yarkony.f90:221-247)
     double factor = 0.0;
     for (int64_t i = 0; i < n; ++i) {
         factor = std::fmax(factor, std::fabs(alpha[i]));
         if (i > 0) factor = std::fmax(factor, std::fabs(beta[i]));
     }
-    double anorm = 0.0;
-    if (n == 1) {
-        root[0] = alpha[0]; v1[0] = 1.0; vn[0] = 1.0;
-    } else if (factor == 0.0) {
-        for (int64_t i = 0; i < n; ++i) { root[i] = 0.0; v1[i] = i == 0; vn[i] = i == n - 1; }
-    } else {
+    double *vec = nullptr;
+    NV_HIP(c, hipMalloc(&vec, (size_t)n * n * sizeof(double)));
+    if (n == 1 || factor == 0.0) {
+        std::vector<double> eye((size_t)n * n, 0.0);
         for (int64_t i = 0; i < n; ++i) {
-            if (i > 0) anorm += (beta[i] / factor) * (beta[i] / factor);
-            anorm += (alpha[i] / factor) * (alpha[i] / factor) / 2.0;
+            eye[(size_t)i * n + i] = 1.0;
+            root[i] = n == 1 ? alpha[0] : 0.0;
         }
-        anorm = std::sqrt(anorm * 2.0) * factor;
-        std::vector<double> d(n), e(n, 0.0), e2(n, 0.0);
-        for (int64_t i = 0; i < n; ++i) d[i] = alpha[i] / anorm;
-        for (int64_t i = 0; i + 1 < n; ++i) { e[i] = beta[i + 1] / anorm; e2[i] = e[i] * e[i]; }
-        nv::TriWork w;
-        std::vector<void *> tmp;
-        auto alloc = [&](size_t bytes, void **p) -> hipError_t {
-            hipError_t er = hipMalloc(p, bytes);
-            if (er == hipSuccess) tmp.push_back(*p);
-            return er;
-        };
-        hipError_t er = alloc(n * 8, (void **)&w.d);
-        if (er == hipSuccess) er = alloc(n * 8, (void **)&w.e);
-        if (er == hipSuccess) er = alloc(n * 8, (void **)&w.e2);
-        if (er == hipSuccess) er = alloc(n * 8, (void **)&w.root);
-        if (er == hipSuccess) er = alloc((size_t)n * n * 8, (void **)&w.vec);
-        if (er == hipSuccess) er = alloc((size_t)4 * n * n * 8, (void **)&w.lu);
-        if (er == hipSuccess) er = alloc(n * 4, (void **)&w.ia);
-        if (er == hipSuccess) er = alloc(n * 8, (void **)&w.rank);
-        if (er == hipSuccess) er = hipMemcpyAsync(w.d, d.data(), n * 8, hipMemcpyHostToDevice, c->stream);
-        if (er == hipSuccess) er = hipMemcpyAsync(w.e, e.data(), n * 8, hipMemcpyHostToDevice, c->stream);
-        if (er == hipSuccess) er = hipMemcpyAsync(w.e2, e2.data(), n * 8, hipMemcpyHostToDevice, c->stream);
-        if (er == hipSuccess) er = nv::launch_tridiag(n, w, anorm, c->stream);
-        if (er == hipSuccess) er = hipMemcpyAsync(root.data(), w.root, n * 8, hipMemcpyDeviceToHost, c->stream);
-        if (er == hipSuccess) er = hipMemcpyAsync(v1.data(), w.vec, n * 8, hipMemcpyDeviceToHost, c->stream);
-        if (er == hipSuccess)
-            er = hipMemcpyAsync(vn.data(), w.vec + (n - 1) * n, n * 8, hipMemcpyDeviceToHost, c->stream);
-        if (er == hipSuccess) er = hipStreamSynchronize(c->stream);
-        for (void *p : tmp) hipFree(p);
-        if (er != hipSuccess) return hip_fail(c, er, "tridiagonal eigensolve");
-        for (int64_t i = 0; i < n; ++i) root[i] *= anorm;
+        hipError_t er = hipMemcpy(vec, eye.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice);
+        if (er != hipSuccess) {
+            hipFree(vec);
+            return hip_fail(c, er, "tridiagonal eigenvectors");
+        }
+        *vec_dev = vec;
+        return NV_OK;
     }
+    double anorm = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i > 0) anorm += (beta[i] / factor) * (beta[i] / factor);
+        anorm += (alpha[i] / factor) * (alpha[i] / factor) / 2.0;
+    }
+    anorm = std::sqrt(anorm * 2.0) * factor;
+    std::vector<double> d(n), e(n, 0.0), e2(n, 0.0);
+    for (int64_t i = 0; i < n; ++i) d[i] = alpha[i] / anorm;
+    for (int64_t i = 0; i + 1 < n; ++i) { e[i] = beta[i + 1] / anorm; e2[i] = e[i] * e[i]; }
+    nv::TriWork w;
+    w.vec = vec;
+    std::vector<void *> tmp;
+    auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+        hipError_t er = hipMalloc(p, bytes);
+        if (er == hipSuccess) tmp.push_back(*p);
+        return er;
+    };
+    hipError_t er = alloc(n * 8, (void **)&w.d);
+    if (er == hipSuccess) er = alloc(n * 8, (void **)&w.e);
+    if (er == hipSuccess) er = alloc(n * 8, (void **)&w.e2);
+    if (er == hipSuccess) er = alloc(n * 8, (void **)&w.root);
+    if (er == hipSuccess) er = alloc((size_t)4 * n * n * 8, (void **)&w.lu);
+    if (er == hipSuccess) er = alloc(n * 4, (void **)&w.ia);
+    if (er == hipSuccess) er = alloc(n * 8, (void **)&w.rank);
+    if (er == hipSuccess) er = hipMemcpyAsync(w.d, d.data(), n * 8, hipMemcpyHostToDevice, c->stream);
+    if (er == hipSuccess) er = hipMemcpyAsync(w.e, e.data(), n * 8, hipMemcpyHostToDevice, c->stream);
+    if (er == hipSuccess) er = hipMemcpyAsync(w.e2, e2.data(), n * 8, hipMemcpyHostToDevice, c->stream);
+    if (er == hipSuccess) er = nv::launch_tridiag(n, w, anorm, c->stream);
+    if (er == hipSuccess) er = hipMemcpyAsync(root.data(), w.root, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (er == hipSuccess) er = hipStreamSynchronize(c->stream);
+    for (void *p : tmp) hipFree(p);
+    if (er != hipSuccess) {
+        hipFree(vec);
+        return hip_fail(c, er, "tridiagonal eigensolve");
+    }
+    for (int64_t i = 0; i < n; ++i) root[i] *= anorm;
+    *vec_dev = vec;
+    return NV_OK;
+}
+
+int32_t nv_ritz(nv_ctx *c, int64_t n, const double *alpha, const double *beta, double *eval, double *inten,
+                double *bji, int32_t *sign) {
+    if (!c) return NV_ERR_ARG;
+    if (n < 1 || !alpha || !beta || !eval) return fail(c, NV_ERR_ARG, "bad tridiagonal arguments");
+    std::vector<double> root, v1(n), vn(n);
+    double *vec = nullptr;
+    int32_t rc = tridiag_solve(c, n, alpha, beta, root, &vec);
+    if (rc != NV_OK) return rc;
+    hipError_t er = hipMemcpyAsync(v1.data(), vec, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (er == hipSuccess) er = hipMemcpyAsync(vn.data(), vec + (n - 1) * n, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (er == hipSuccess) er = hipStreamSynchronize(c->stream);
+    hipFree(vec);
+    if (er != hipSuccess) return hip_fail(c, er, "tridiagonal eigenvectors");
     const double bn = beta[n];
     for (int64_t i = 0; i < n; ++i) {
         eval[i] = root[i];
@@ -669,6 +699,107 @@ int32_t nv_ritz(nv_ctx *c, int64_t n, const double *alpha, const double *beta, d
         if (bji) bji[i] = std::fabs(bn * vn[i]);
         if (sign) sign[i] = v1[i] < 0 ? -1 : 1;
     }
+    return NV_OK;
+}
+
+// ---- Ritz vectors and root analysis -------------------------------------------
+int32_t nv_identify_roots(nv_ctx *c, int64_t n, const double *alpha, const double *beta, int64_t nroots,
+                          int64_t idroots, double *eval, double *inten, int32_t *sign, int64_t *top_index,
+                          int32_t *top_state, int32_t *top_phase, double *top_mag2, double *percent, double *overlap,
+                          double *vectors) {
+    if (!c) return NV_ERR_ARG;
+    if (!alpha || !beta) return fail(c, NV_ERR_ARG, "alpha/beta missing");
+    if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
+    if (c->opts.reorthog <= 0 && c->opts.savevecs == 0)
+        return fail(c, NV_ERR_STATE, "Krylov basis not kept (set savevecs or reorthog)");
+    if (n < 1 || n > c->last_iter) return fail(c, NV_ERR_ARG, "n must lie in 1..last iteration");
+    if (nroots < 1 || nroots > n) return fail(c, NV_ERR_ARG, "nroots must lie in 1..n");
+    const int64_t ns = c->op.ns, D = c->op.D, N = c->op.N;
+    if (nroots * ns > 4096) return fail(c, NV_ERR_ARG, "nroots * nstates must not exceed 4096");
+    if (idroots < 0 || idroots > N) return fail(c, NV_ERR_ARG, "idroots must lie in 0..n");
+    if (idroots > 0 && (!top_index || !top_state || !top_phase || !top_mag2))
+        return fail(c, NV_ERR_ARG, "top_* outputs missing");
+    std::vector<double> root;
+    double *vec = nullptr;
+    int32_t rc = tridiag_solve(c, n, alpha, beta, root, &vec);
+    if (rc != NV_OK) return rc;
+    std::vector<void *> tmp{vec};
+    auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+        hipError_t er = hipMalloc(p, bytes);
+        if (er == hipSuccess) tmp.push_back(*p);
+        return er;
+    };
+    const int R = (int)(nroots * ns);
+    int npairs = 0, nchunks = 0;
+    int64_t rows = 0;
+    nv::gram_shape(D, R, npairs, nchunks, rows);
+    const int nb = nv::topk_blocks(N);
+    const int64_t kk = idroots > 0 ? idroots : 1;
+    double *Y = nullptr, *G = nullptr, *part = nullptr, *pm = nullptr, *prev_m = nullptr, *om = nullptr;
+    int64_t *pi = nullptr, *prev_i = nullptr, *oi = nullptr;
+    int32_t *oph = nullptr;
+    std::vector<double> v1(n), gram((size_t)R * R);
+    // q_1 .. q_n are slots 1..n of the archive (nslots = niter + 2)
+    const double *V = c->st.V + N;
+    hipError_t er = alloc((size_t)nroots * N * 8, (void **)&Y);
+    if (er == hipSuccess) er = alloc((size_t)R * R * 8, (void **)&G);
+    if (er == hipSuccess) er = alloc((size_t)nchunks * npairs * nv::kGramT * nv::kGramT * 8, (void **)&part);
+    if (er == hipSuccess) er = alloc((size_t)nroots * nb * 8, (void **)&pm);
+    if (er == hipSuccess) er = alloc((size_t)nroots * nb * 8, (void **)&pi);
+    if (er == hipSuccess) er = alloc((size_t)nroots * 8, (void **)&prev_m);
+    if (er == hipSuccess) er = alloc((size_t)nroots * 8, (void **)&prev_i);
+    if (er == hipSuccess) er = alloc((size_t)nroots * kk * 8, (void **)&om);
+    if (er == hipSuccess) er = alloc((size_t)nroots * kk * 8, (void **)&oi);
+    if (er == hipSuccess) er = alloc((size_t)nroots * kk * 4, (void **)&oph);
+    if (er == hipSuccess) er = hipMemcpyAsync(v1.data(), vec, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (er == hipSuccess) er = nv::launch_ritz_vectors(V, N, n, vec, nroots, Y, c->stream);
+    if (er == hipSuccess && idroots > 0)
+        er = nv::launch_topk(Y, N, (int)nroots, (int)idroots, prev_m, prev_i, pm, pi, om, oi, oph, c->stream);
+    if (er == hipSuccess) er = nv::launch_gram(Y, D, R, part, G, c->stream);
+    if (er == hipSuccess) er = hipMemcpyAsync(gram.data(), G, (size_t)R * R * 8, hipMemcpyDeviceToHost, c->stream);
+    std::vector<double> hm((size_t)nroots * kk);
+    std::vector<int64_t> hi((size_t)nroots * kk);
+    std::vector<int32_t> hph((size_t)nroots * kk);
+    if (er == hipSuccess && idroots > 0) {
+        er = hipMemcpyAsync(hm.data(), om, hm.size() * 8, hipMemcpyDeviceToHost, c->stream);
+        if (er == hipSuccess) er = hipMemcpyAsync(hi.data(), oi, hi.size() * 8, hipMemcpyDeviceToHost, c->stream);
+        if (er == hipSuccess) er = hipMemcpyAsync(hph.data(), oph, hph.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    }
+    if (er == hipSuccess && vectors)
+        er = hipMemcpyAsync(vectors, Y, (size_t)nroots * N * 8, hipMemcpyDeviceToHost, c->stream);
+    if (er == hipSuccess) er = hipStreamSynchronize(c->stream);
+    for (void *p : tmp) hipFree(p);
+    if (er != hipSuccess) return hip_fail(c, er, "identify_roots");
+    // intensities relative to the strongest of the nroots roots (Source/top.f90:1165-1168)
+    double intfactor = 0.0;
+    for (int64_t i = 0; i < nroots; ++i) intfactor = std::fmax(intfactor, v1[i] * v1[i]);
+    for (int64_t i = 0; i < nroots; ++i) {
+        if (eval) eval[i] = root[i];
+        if (inten) inten[i] = intfactor > 0.0 ? v1[i] * v1[i] / intfactor : 0.0;
+        if (sign) sign[i] = v1[i] < 0 ? -1 : 1;
+    }
+    for (int64_t i = 0; i < nroots && idroots > 0; ++i)
+        for (int64_t k = 0; k < idroots; ++k) {
+            const size_t o = (size_t)i * idroots + k;
+            const int64_t e = hi[o];
+            top_index[o] = e >= 0 ? e % D + 1 : 0;
+            top_state[o] = e >= 0 ? (int32_t)(e / D + 1) : 0;
+            top_phase[o] = hph[o];
+            top_mag2[o] = hm[o];
+        }
+    if (percent)
+        for (int64_t i = 0; i < nroots; ++i) {
+            double tot = 0.0;
+            for (int64_t k = 0; k < ns; ++k) tot += gram[(size_t)(i * ns + k) * R + i * ns + k];
+            for (int64_t k = 0; k < ns; ++k)
+                percent[i * ns + k] = 100.0 * std::fabs(gram[(size_t)(i * ns + k) * R + i * ns + k] / tot);
+        }
+    if (overlap)
+        for (int64_t i = 0; i < nroots; ++i)
+            for (int64_t j = 0; j < nroots; ++j)
+                for (int64_t k = 0; k < ns; ++k)
+                    for (int64_t l = 0; l < ns; ++l)
+                        overlap[((i * nroots + j) * ns + k) * ns + l] = gram[(size_t)(i * ns + k) * R + j * ns + l];
     return NV_OK;
 }
 
@@ -722,7 +853,8 @@ int32_t nv_load_vectors(nv_ctx *c, const char *path, int64_t iter, const double 
     if (!c || !path || !alpha || !beta) return NV_ERR_ARG;
     if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
     if (iter < 1 || iter >= c->opts.niter) return fail(c, NV_ERR_ARG, "restart iteration out of range");
-    if (c->opts.reorthog > 0) return fail(c, NV_ERR_STATE, "restart with re-orthogonalisation needs the archive");
+    if (c->opts.reorthog > 0 || c->opts.savevecs != 0)
+        return fail(c, NV_ERR_STATE, "a restart cannot rebuild the archive of earlier Lanczos vectors");
     (void)omega;
     const int64_t N = c->op.N;
     std::vector<double> buf(3 * N);

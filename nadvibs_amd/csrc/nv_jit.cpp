@@ -239,7 +239,7 @@ hipError_t jit_build(const Operator &op, int prec, JitKernels &out, std::string 
 }
 
 void jit_release(JitKernels &k) {
-    if (k.module) hipModuleUnload(k.module);
+    if (k.module) (void)hipModuleUnload(k.module);
     k = JitKernels{};
 }
 

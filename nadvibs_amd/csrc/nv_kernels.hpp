@@ -64,11 +64,29 @@ struct TriWork {
     double *e = nullptr;     // [n] normalised off-diagonal, e[j] couples j, j+1
     double *e2 = nullptr;    // [n] squares
     double *root = nullptr;  // [n]
-    double *vec = nullptr;   // [n][n], vec[i*n + j] = component j of root i
+    double *vec = nullptr;   // [n][n], vec[j*n + i] = component j of root i
     double *lu = nullptr;    // [5][n][n] per-root elimination rows, interleaved by root
     int32_t *ia = nullptr;   // [n] position inside a cluster of near-equal roots
     int64_t *rank = nullptr; // [n] number of cluster members (ia>0) before root i
 };
 hipError_t launch_tridiag(int64_t n, const TriWork &w, double anorm, hipStream_t s);
+
+// ---- Ritz vectors and root analysis (nv_roots.hip) ----
+constexpr int kRitzChunk = 16;  // Ritz vectors formed per pass over the basis
+constexpr int kRitzKB = 64;     // basis vectors per LDS stage of the eigenvectors
+constexpr int kGramT = 32;      // Gram tile edge
+// Y[i][r] = sum_k vec[k*n + i] V[k*N + r], i < nroots, k < n.
+hipError_t launch_ritz_vectors(const double *V, int64_t N, int64_t n, const double *vec, int64_t nroots, double *Y,
+                               hipStream_t s);
+// Partial-result blocks per root of launch_topk (pm, pi hold nroots * topk_blocks(N)).
+int topk_blocks(int64_t N);
+// For each of nroots vectors Y + root*N: the kmax largest squared elements in
+// rank order (ties by index), their indices and signs.  out_* are [nroots][kmax].
+hipError_t launch_topk(const double *Y, int64_t N, int nroots, int kmax, double *prev_m, int64_t *prev_i, double *pm,
+                       int64_t *pi, double *out_m, int64_t *out_i, int32_t *out_ph, hipStream_t s);
+// Partial-tile layout of launch_gram: part holds nchunks * npairs * kGramT^2 doubles.
+void gram_shape(int64_t D, int R, int &npairs, int &nchunks, int64_t &rows_per_chunk);
+// G (R x R) = A^T A, column c of A at A + c*D.
+hipError_t launch_gram(const double *A, int64_t D, int R, double *part, double *G, hipStream_t s);
 
 }  // namespace nv

@@ -106,9 +106,10 @@ class Engine:
         return q
 
     # -- Lanczos ---------------------------------------------------------------
-    def lanczos_init(self, niter, reorthog=0, prec="f64", chkorthog=100, seed=12345, noise=1.0, pro_eps=0.0):
+    def lanczos_init(self, niter, reorthog=0, prec="f64", chkorthog=100, seed=12345, noise=1.0, pro_eps=0.0,
+                     savevecs=False):
         o = _lib.NvLanczosOpts(niter=niter, reorthog=reorthog, prec=_lib.PREC[prec], chkorthog=chkorthog,
-                               seed=seed, noise_scale=noise, pro_eps=pro_eps)
+                               seed=seed, noise_scale=noise, pro_eps=pro_eps, savevecs=1 if savevecs else 0)
         self._check(self.lib.nv_lanczos_init(self.h, ctypes.byref(o)))
         self.niter = niter
         self.reorthog = reorthog
@@ -161,6 +162,37 @@ class Engine:
         self._check(self.lib.nv_ritz(self.h, n, dptr(alpha), dptr(beta), dptr(ev), dptr(it), dptr(bj),
                                      sg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return ev, it, bj, sg
+
+    def identify_roots(self, alpha, beta, nroots, idroots=0, vectors=False):
+        """Ritz vectors of the lowest nroots roots and their analysis
+        (identify_roots, Source/top.f90:1122-1283).  Needs the Krylov basis
+        (lanczos_init with savevecs or reorthog).  Returns a dict with eval,
+        inten (relative to the strongest of the nroots), sign, top_index
+        (1-based vibrational index), top_state (1-based), top_phase, top_mag2
+        ([nroots, idroots]), percent [nroots, nstates], overlap
+        [nroots, nroots, nstates, nstates] and, if asked, vectors [nroots, N]."""
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        n = alpha.size
+        if beta.size != n + 1:
+            raise ValueError("beta must hold beta(0:n)")
+        ns = self.prob.nstates
+        k = max(idroots, 1)
+        out = dict(eval=np.empty(nroots), inten=np.empty(nroots), sign=np.empty(nroots, dtype=np.int32),
+                   top_index=np.zeros((nroots, k), dtype=np.int64), top_state=np.zeros((nroots, k), dtype=np.int32),
+                   top_phase=np.zeros((nroots, k), dtype=np.int32), top_mag2=np.zeros((nroots, k)),
+                   percent=np.empty((nroots, ns)), overlap=np.empty((nroots, nroots, ns, ns)))
+        vec = np.empty((nroots, self.n)) if vectors else None
+        i32 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        self._check(self.lib.nv_identify_roots(
+            self.h, n, dptr(alpha), dptr(beta), nroots, idroots, dptr(out["eval"]), dptr(out["inten"]),
+            i32(out["sign"]), out["top_index"].ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), i32(out["top_state"]),
+            i32(out["top_phase"]), dptr(out["top_mag2"]), dptr(out["percent"]), dptr(out["overlap"]), dptr(vec)))
+        for key in ("top_index", "top_state", "top_phase", "top_mag2"):
+            out[key] = out[key][:, :idroots]
+        if vectors:
+            out["vectors"] = vec
+        return out
 
     def hv_kernel(self, prec="f64"):
         """Which H*v kernel serves `prec` (compiles it on first use)."""
@@ -231,6 +263,74 @@ def eigen_table(prob, evals, inten, bji, bconv=0):
                                                      (evals[k] - evals[0]) * AU2EV, inten[k] / imax))
     lines += ["", "   All requested eigenvalues computed."]
     return "\n".join(lines) + "\n"
+
+
+def _es(v, w, d):
+    """Fortran ESw.d."""
+    s = "%.*E" % (d, v)
+    mant, ex = s.split("E")
+    e = int(ex)
+    s = f"{mant}E{'-' if e < 0 else '+'}{abs(e):02d}" if abs(e) < 100 else f"{mant}{'-' if e < 0 else '+'}{abs(e):03d}"
+    return s.rjust(w)
+
+
+def barray(prob, index):
+    """0-based quantum numbers of a 1-based vibrational index (get_barray,
+    Source/secondary.f90:812-824; the last mode varies fastest)."""
+    lab = max(int(index) - 1, 0)
+    out = [0] * prob.nmodes
+    for m in range(prob.nmodes - 1, -1, -1):
+        out[m] = lab % prob.nfunc[m]
+        lab //= prob.nfunc[m]
+    return out
+
+
+def rootinfo_text(prob, res):
+    """rootinfo.dat as identify_roots writes it (Source/top.f90:1158-1225,
+    formats :1266-1278), from the dict of Engine.identify_roots."""
+    nroots, ns = res["percent"].shape
+    idroots = res["top_index"].shape[1]
+    out = [" Summary for computing %4d eigenvectors corresponding to the lowest eigenvalues" % nroots]
+    for i in range(nroots):
+        out += [" Eigenvector %4d has been computed..." % (i + 1), "", " ************ ROOT %4d *************" % (i + 1),
+                " Energy: %9.2f cm-1, %9.5f eV   Intensity: %6.4f" % (res["eval"][i] * AU2WAVE,
+                                                                     res["eval"][i] * AU2EV, res["inten"][i]), ""]
+        for j in range(idroots):
+            out.append(" State vector: %2d | Index: %12d Phase: %2d | Magnitude^2: %8.6f" % (
+                res["top_state"][i, j], res["top_index"][i, j], res["top_phase"][i, j], res["top_mag2"][i, j]))
+            out.append(" v(i): " + "".join("%3d" % b for b in barray(prob, res["top_index"][i, j])))
+        out.append(" ")
+        if 2 <= ns <= 5:
+            lab = ",".join(str(k + 1) for k in range(ns))
+            out.append(" Percent State (%s): (" % lab + ", ".join("%6.2f %%" % v for v in res["percent"][i]) + ")")
+        out.append(" --------------------------------------------------------------")
+    out += [" ", "------------ FINISHED EXECUTION ------------"]
+    return "\n".join(out) + "\n"
+
+
+def sodata_text(res, soroots):
+    """sodata.in as identify_roots writes it (Source/top.f90:1169-1179,
+    :1227-1243, formats :1260-1265, :1279-1280)."""
+    ns = res["percent"].shape[1]
+    out = ["%5d" % soroots, " Root   Energy (eV)      Sqrt(Int.)*sign"]
+    for i in range(soroots):
+        out.append("%5d  %s  %s" % (i + 1, _es(res["eval"][i] * AU2EV, 15, 8),
+                                    _es(np.sqrt(res["inten"][i]) * res["sign"][i], 15, 8)))
+    if ns == 2:
+        out.append(" R(i)  R(j)   [S1(i)S2(j)]   [S2(i)S1(j)]")
+    if ns == 3:
+        out += [" R(i)  R(j)   [S1(i)S2(j)]   [S1(i)S3(j)]  [S2(i)S1(j)]",
+                " R(i)  R(j)   [S2(i)S3(j)]   [S3(i)S1(j)]  [S3(i)S2(j)]"]
+    for i in range(soroots):
+        for j in range(i):
+            dp = [res["overlap"][i, j, k, l] for k in range(ns) for l in range(ns) if k != l]
+            if ns == 2:
+                out.append("%5d %5d  %s %s" % (i + 1, j + 1, _es(dp[0], 14, 6), _es(dp[1], 14, 6)))
+            if ns == 3:
+                for h in (0, 3):
+                    out.append("%5d %5d  %s %s %s" % (i + 1, j + 1, _es(dp[h], 14, 6), _es(dp[h + 1], 14, 6),
+                                                      _es(dp[h + 2], 14, 6)))
+    return "\n".join(out) + "\n"
 
 
 def write_restart_log(path, alpha, beta, omega=None):

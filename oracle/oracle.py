@@ -174,3 +174,36 @@ def givens(alpha, beta, vectors=False):
     if vectors:
         return ev, it, bj, sg, vec.reshape(n, n)
     return ev, it, bj, sg
+
+
+def identify_roots(alpha, beta, basis, nroots, idroots, nstates):
+    """Ritz vectors and root analysis restated (identify_roots,
+    Source/top.f90:1122-1283): Y_i = sum_k v(k,i) q_k (:1177-1183), the
+    idroots largest |Y_i|^2 elements with their signs, ties by increasing
+    Fortran index (:1191-1200), state weights in percent (:1203-1210) and the
+    overlaps Y_i[:,k] . Y_j[:,l] (:1227-1243).  basis holds q_1..q_n as rows."""
+    alpha = np.asarray(alpha, dtype=np.float64)
+    n = alpha.size
+    ev, inten, bji, sg, vec = givens(alpha, beta, vectors=True)
+    Y = vec[:nroots, :n] @ np.asarray(basis)[:n]
+    N = Y.shape[1]
+    D = N // nstates
+    imax = inten[:nroots].max()
+    out = dict(eval=ev[:nroots].copy(), inten=inten[:nroots] / imax, sign=sg[:nroots].copy(), vectors=Y)
+    ti = np.zeros((nroots, idroots), dtype=np.int64)
+    ts = np.zeros((nroots, idroots), dtype=np.int32)
+    tp = np.zeros((nroots, idroots), dtype=np.int32)
+    tm = np.zeros((nroots, idroots))
+    for i in range(nroots):
+        m = Y[i] * Y[i]
+        order = np.lexsort((np.arange(N), -m))[:idroots]
+        ti[i] = order % D + 1
+        ts[i] = order // D + 1
+        tp[i] = np.where(Y[i, order] < 0, -1, 1)
+        tm[i] = m[order]
+    out.update(top_index=ti, top_state=ts, top_phase=tp, top_mag2=tm)
+    Y3 = Y.reshape(nroots, nstates, D)
+    w = np.einsum("ikr,ikr->ik", Y3, Y3)
+    out["percent"] = 100.0 * np.abs(w / w.sum(axis=1, keepdims=True))
+    out["overlap"] = np.einsum("ikr,jlr->ijkl", Y3, Y3)
+    return out
