@@ -147,6 +147,7 @@ program nadvibs_main
     opts%seed = 20240101_c_int64_t; opts%noise_scale = 1d0; opts%pro_eps = 0d0
     ! savevecs (Source/secondary.f90:863): the root analysis reads every Lanczos vector
     opts%savevecs = 0
+    opts%checkpoint = 0
     if (idroots > 0 .or. soroots > 0) opts%savevecs = 1
     if (restart /= 0 .and. opts%savevecs /= 0) &
         stop 'nadvibs: idroots/soroots need every Lanczos vector and cannot follow a restart'

@@ -24,6 +24,7 @@ module nadvibs_abi
         integer(c_int64_t) :: seed
         real(c_double) :: noise_scale, pro_eps
         integer(c_int32_t) :: savevecs
+        integer(c_int64_t) :: checkpoint
     end type nv_lanczos_opts
 
     interface

@@ -135,6 +135,10 @@ typedef struct nv_lanczos_opts {
                                  reorthog > 0, idroots > 0 or soroots > 0,
                                  Source/secondary.f90:863); needed by
                                  nv_identify_roots.  reorthog > 0 implies it. */
+    int64_t checkpoint;       /* K > 0 with neither savevecs nor reorthog: keep
+                                 (q_{cK}, r_{cK}) every K iterations so that
+                                 nv_lanczos_adjoint can recompute the basis
+                                 segment by segment; 0 = none */
 } nv_lanczos_opts;
 
 /* Allocates the Krylov basis for opts->niter iterations, builds the start
@@ -199,6 +203,43 @@ int32_t nv_identify_roots(nv_ctx *ctx, int64_t n, const double *alpha,
                           int64_t *top_index, int32_t *top_state,
                           int32_t *top_phase, double *top_mag2,
                           double *percent, double *overlap, double *vectors);
+
+/* ---- derivatives (no counterpart in the reference: gradients of a
+ *      spectrum loss for fitting the coupling constants) ------------------ */
+/* Reverse sweep of the recurrence of the last nv_lanczos_run /
+ * per-iteration calls (reorthog = 0, default start vector): given
+ * abar = dL/dalpha(1:n) and bbar = dL/dbeta(0:n), accumulates
+ *   g_omega[nmodes]      dL/daomega (hartree),
+ *   g_concoef[nstblks]   dL/dconcoef,
+ *   g_potterms[...]      dL/dPOTterms, same layout as nv_problem.potterms
+ *                        (every slot, also coefficients below ztoler),
+ * in FP64 with H*v in precision prec.  Needs the Krylov basis (savevecs) or
+ * checkpoints (opts.checkpoint); with checkpoints each segment of K
+ * iterations is recomputed on the device and checked bit for bit against the
+ * stored alpha/beta.  Any output may be NULL.  Synchronises. */
+int32_t nv_lanczos_adjoint(nv_ctx *ctx, int64_t n, const double *abar,
+                           const double *bbar, int32_t prec, double *g_omega,
+                           double *g_concoef, double *g_potterms);
+/* env[g] = sum_i v(1,i)^2 exp(-((grid[g] - (theta_i - ref) AU2WAVE)/sigma)^2/2)
+ * for the eigenpairs (theta_i, v(:,i)) of T(alpha, beta): the broadened
+ * stick spectrum (convolve.py:28-48 without the final division by the
+ * maximum), grid and sigma in cm^-1, ref = theta_1 (origin = 0, as the
+ * eigenvalue table of Source/top.f90:1087-1102) or e0 hartree (origin = 1). */
+int32_t nv_spectrum(nv_ctx *ctx, int64_t n, const double *alpha,
+                    const double *beta, int64_t ng, const double *grid,
+                    double sigma, int32_t origin, double e0, double *env);
+/* Given gbar = dL/denv: abar = dL/dalpha(1:n), bbar = dL/dbeta(0:n) (bbar[0]
+ * = bbar[n] = 0), by divided differences of the eigenvalues (finite for
+ * near-degenerate Ritz values). */
+int32_t nv_spectrum_adjoint(nv_ctx *ctx, int64_t n, const double *alpha,
+                            const double *beta, int64_t ng,
+                            const double *grid, double sigma, int32_t origin,
+                            double e0, const double *gbar, double *abar,
+                            double *bbar);
+/* generic != 0: use the generic H*v kernel instead of compiling an
+ * operator-specialised one (saves the run-time compile for operators used
+ * once, e.g. inside a fit).  Call before nv_lanczos_init. */
+int32_t nv_select_hv_kernel(nv_ctx *ctx, int32_t generic);
 
 /* ---- spectrum broadening (convolve.py:28-48, the Gaussian aperture) ----- */
 /* env[g] = sum_j y[j] exp(-((grid[g] - x[j]) / sigma)^2 / 2), divided by its

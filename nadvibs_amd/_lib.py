@@ -40,7 +40,7 @@ class NvLanczosOpts(ctypes.Structure):
     _fields_ = [
         ("niter", ctypes.c_int64), ("reorthog", ctypes.c_int32), ("prec", ctypes.c_int32),
         ("chkorthog", ctypes.c_int64), ("seed", ctypes.c_uint64), ("noise_scale", ctypes.c_double),
-        ("pro_eps", ctypes.c_double), ("savevecs", ctypes.c_int32),
+        ("pro_eps", ctypes.c_double), ("savevecs", ctypes.c_int32), ("checkpoint", ctypes.c_int64),
     ]
 
 
@@ -67,6 +67,13 @@ SIGNATURES = [
     ("nv_pro_stats", ctypes.c_int32, [ctypes.c_void_p, _i64p, _i64p]),
     ("nv_get_lanczos_vector", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp]),
     ("nv_ritz", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _i32p]),
+    ("nv_lanczos_adjoint", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int32, _dp, _dp,
+                                            _dp]),
+    ("nv_spectrum", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp, ctypes.c_double,
+                                     ctypes.c_int32, ctypes.c_double, _dp]),
+    ("nv_spectrum_adjoint", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp,
+                                             ctypes.c_double, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp]),
+    ("nv_select_hv_kernel", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     ("nv_identify_roots", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
                                            _dp, _dp, _i32p, _i64p, _i32p, _i32p, _dp, _dp, _dp, _dp]),
     ("nv_convolve", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp,

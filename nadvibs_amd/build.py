@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnadvibs.so")
 
-SOURCES = ["nv_operator.cpp", "nv_jit.cpp", "nv_kernels.hip", "nv_pro.hip", "nv_fcstart.hip", "nv_roots.hip",
+SOURCES = ["nv_operator.cpp", "nv_jit.cpp", "nv_kernels.hip", "nv_pro.hip", "nv_fcstart.hip", "nv_roots.hip", "nv_adjoint.hip",
            "nv_api.hip"]
 
 # Packed FP32 instructions are disabled for every device pass.

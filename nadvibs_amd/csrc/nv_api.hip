@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -48,6 +49,19 @@ struct nv_ctx {
     nv::JitKernels jit[2];
     std::string jit_log;
     float *hx32 = nullptr;
+
+    // raw nadvibs.in coefficients (for the frequency chain rule of the adjoint)
+    std::vector<double> potraw;
+    double ztoler = 0.0;
+    int64_t ntmax = 0;
+    // checkpoints (q_{cK}, r_{cK}) of a run without the archive, c = 0..ncp-1
+    double *cp = nullptr;
+    int64_t ncp = 0;
+    // gradient operator: every monomial with a unit coefficient (adjoint)
+    bool gop_ready = false;
+    nv::Operator gop;
+    NvOpDev gdev{};
+    std::vector<void *> gallocs;
 
     // H*v timing
     bool prof = false;
@@ -103,6 +117,8 @@ void release_lanczos(nv_ctx *c) {
     release_graphs(c);
     if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
     free_list(c->lz_allocs);
+    c->cp = nullptr;
+    c->ncp = 0;
     c->st = nv::LzState{};
     c->pro = nv::ProState{};
     c->lz_ready = false;
@@ -126,6 +142,9 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
         return fail(nullptr, NV_ERR_ARG, err);
     }
     c->device = device;
+    c->ntmax = nv::noterms(prob->nmodes, prob->ordr);
+    c->potraw.assign(prob->potterms, prob->potterms + c->ntmax * prob->ordr * c->op.nstblks);
+    c->ztoler = prob->ztoler;
     {
         const char *k = std::getenv("NV_HV_KERNEL");
         c->jit_on = !(k && std::strcmp(k, "generic") == 0);
@@ -199,6 +218,7 @@ int32_t nv_destroy(nv_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     release_lanczos(c);
     for (auto &k : c->jit) nv::jit_release(k);
+    free_list(c->gallocs);
     if (c->hx32) hipFree(c->hx32);
     free_list(c->allocs);
     for (auto &p : c->ev) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
@@ -251,12 +271,31 @@ static const nv::JitKernels *ensure_jit(nv_ctx *c, int32_t prec) {
     return &k;
 }
 
-static hipError_t launch_hv_iter_any(nv_ctx *c) {
+static hipError_t launch_hv_iter_st(nv_ctx *c, const nv::LzState &st) {
     const nv::JitKernels *k = ensure_jit(c, c->opts.prec);
-    if (!k) return nv::launch_hv_iter(c->dev, c->opts.prec, c->st, c->stream);
-    hipError_t e = nv::jit_launch_iter(*k, c->dev, c->st, c->stream);
+    if (!k) return nv::launch_hv_iter(c->dev, c->opts.prec, st, c->stream);
+    hipError_t e = nv::jit_launch_iter(*k, c->dev, st, c->stream);
     if (e != hipSuccess) return e;
-    return nv::launch_finish_sum(c->st.partials, nv::hv_blocks(c->dev), c->st.alpha, c->st.iter, 0, c->stream);
+    return nv::launch_finish_sum(st.partials, nv::hv_blocks(c->dev), st.alpha, st.iter, 0, c->stream);
+}
+
+static hipError_t launch_hv_iter_any(nv_ctx *c) { return launch_hv_iter_st(c, c->st); }
+
+// y = H x (double in and out) on stream s, couplings in precision prec.
+static hipError_t hv_apply(nv_ctx *c, int32_t prec, const double *x, double *y, hipStream_t s) {
+    const nv::JitKernels *k = ensure_jit(c, prec);
+    if (!k) return nv::launch_hv_full(c->dev, prec, x, y, s);
+    const void *xin = x;
+    if (prec == NV_PREC_F32) {
+        if (!c->hx32) {
+            hipError_t e = hipMalloc(&c->hx32, (size_t)c->op.N * sizeof(float));
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = nv::launch_cast_f32(x, c->hx32, c->op.N, s);
+        if (e != hipSuccess) return e;
+        xin = c->hx32;
+    }
+    return nv::jit_launch_full(*k, c->dev, x, xin, y, s);
 }
 
 int32_t nv_hv_device(nv_ctx *c, int32_t prec, const double *x, double *y, void *stream) {
@@ -266,18 +305,15 @@ int32_t nv_hv_device(nv_ctx *c, int32_t prec, const double *x, double *y, void *
         return fail(c, NV_ERR_ARG, "nv_hv supports NV_PREC_F64 and NV_PREC_F32");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     NV_HIP(c, hipSetDevice(c->device));
-    const nv::JitKernels *k = ensure_jit(c, prec);
-    if (!k) {
-        NV_HIP(c, nv::launch_hv_full(c->dev, prec, x, y, s));
-        return NV_OK;
-    }
-    const void *xin = x;
-    if (prec == NV_PREC_F32) {
-        if (!c->hx32) NV_HIP(c, hipMalloc(&c->hx32, (size_t)c->op.N * sizeof(float)));
-        NV_HIP(c, nv::launch_cast_f32(x, c->hx32, c->op.N, s));
-        xin = c->hx32;
-    }
-    NV_HIP(c, nv::jit_launch_full(*k, c->dev, x, xin, y, s));
+    NV_HIP(c, hv_apply(c, prec, x, y, s));
+    return NV_OK;
+}
+
+int32_t nv_select_hv_kernel(nv_ctx *c, int32_t generic) {
+    if (!c) return NV_ERR_ARG;
+    const bool on = generic == 0;
+    if (on != c->jit_on) release_graphs(c);  // captured iterations name the kernel
+    c->jit_on = on;
     return NV_OK;
 }
 
@@ -356,7 +392,10 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     const int64_t N = c->op.N;
     const bool keep = o->reorthog > 0 || o->savevecs != 0;
     // buffers of an earlier run with the same shape are reused
+    if (o->checkpoint < 0) return fail(c, NV_ERR_ARG, "checkpoint must be >= 0");
+    const int64_t K = keep ? 0 : o->checkpoint;
     const bool same = c->lz_ready && c->opts.niter == o->niter && c->opts.prec == o->prec &&
+                      (keep ? 0 : c->opts.checkpoint) == K &&
                       (c->opts.reorthog > 0 || c->opts.savevecs != 0) == keep &&
                       (c->opts.reorthog > 0) == (o->reorthog > 0) && (c->opts.reorthog > 1) == (o->reorthog > 1);
     nv::LzState &st = c->st;
@@ -381,6 +420,10 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
         NV_HIP(c, alloc(nv::kNumCounters * sizeof(unsigned), (void **)&st.counters));
         NV_HIP(c, alloc(sizeof(int64_t), (void **)&c->d_iter));
         st.iter = c->d_iter;
+        if (K > 0) {
+            c->ncp = (o->niter + K - 1) / K;
+            NV_HIP(c, alloc((size_t)c->ncp * 2 * N * sizeof(double), (void **)&c->cp));
+        }
     }
     c->opts = *o;
     if (c->opts.chkorthog < 1) c->opts.chkorthog = 100;
@@ -399,6 +442,11 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     int32_t rc = build_start(c, s1);
     if (rc != NV_OK) return rc;
     NV_HIP(c, nv::launch_norm_to(s1, N, st.beta, st.partials, st.counters + nv::kCntMisc, c->stream));
+    if (c->cp) {
+        // checkpoint 0: q_0 = 0 and the unnormalised start vector
+        NV_HIP(c, hipMemsetAsync(c->cp, 0, (size_t)N * sizeof(double), c->stream));
+        NV_HIP(c, hipMemcpyAsync(c->cp + N, s1, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
     c->lz_ready = true;
     c->last_iter = 0;
     c->next_update = 1;
@@ -475,12 +523,27 @@ int32_t nv_profile_read(nv_ctx *c, double *ms, int64_t *count) {
     return NV_OK;
 }
 
+// After iteration it of a run with checkpoints: keep (q_it, r_it) every K iterations.
+static hipError_t save_checkpoint(nv_ctx *c, int64_t it) {
+    const int64_t K = c->opts.checkpoint;
+    if (!c->cp || it % K != 0 || it / K >= c->ncp) return hipSuccess;
+    const int64_t N = c->op.N;
+    double *dst = c->cp + (size_t)(it / K) * 2 * N;
+    hipError_t e = hipMemcpyAsync(dst, c->st.V + (it % c->st.nslots) * N, N * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dst + N, c->st.V + ((it + 1) % c->st.nslots) * N, N * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->stream);
+    return e;
+}
+
 int32_t nv_lanczos_step(nv_ctx *c, int64_t it) {
     int32_t rc = check_iter(c, it);
     if (rc) return rc;
     if (it != c->next_update - 1) return fail(c, NV_ERR_STATE, "lanczos_step called before update_lanczos");
     NV_HIP(c, nv::launch_lanczos_step(c->dev, c->opts.prec, c->st, c->stream));
     c->last_iter = it;
+    NV_HIP(c, save_checkpoint(c, it));
     return NV_OK;
 }
 
@@ -560,6 +623,7 @@ int32_t nv_lanczos_run(nv_ctx *c, int64_t iiter, int64_t niter) {
         NV_HIP(c, hipGraphLaunch(ge, c->stream));
         c->next_update = i + 1;
         c->last_iter = i;
+        NV_HIP(c, save_checkpoint(c, i));
     }
     return NV_OK;
 }
@@ -803,6 +867,278 @@ int32_t nv_identify_roots(nv_ctx *c, int64_t n, const double *alpha, const doubl
     return NV_OK;
 }
 
+// ---- derivatives -----------------------------------------------------------------
+// The gradient operator: every monomial of the basis with a unit coefficient
+// in every block, so that its term list covers each nadvibs.in slot.
+static int32_t ensure_gop(nv_ctx *c) {
+    if (c->gop_ready) return NV_OK;
+    const nv::Operator &op = c->op;
+    std::vector<double> ones((size_t)c->ntmax * op.ordr * op.nstblks, 1.0), cc(op.nstblks, 1.0);
+    nv_problem pr{};
+    pr.nstates = op.ns; pr.nmodes = op.M; pr.ordr = op.ordr;
+    pr.nfunc = op.nfunc.data(); pr.aomega = op.omega.data(); pr.concoef = cc.data(); pr.potterms = ones.data();
+    pr.ztoler = 0.0; pr.quirks = op.quirks; pr.neworigin = 0;
+    std::string err;
+    if (!nv::build_operator(pr, c->gop, err)) return fail(c, NV_ERR_ARG, "gradient operator: " + err);
+    const nv::Operator &g = c->gop;
+    NvOpDev &d = c->gdev;
+    d = NvOpDev{};
+    d.ns = (int32_t)g.ns; d.M = (int32_t)g.M; d.D = (int32_t)g.D; d.N = (int32_t)g.N;
+    d.nmax = (int32_t)g.nmax; d.ncodes = g.ncodes;
+    d.nterms = (int32_t)g.dev_terms.size(); d.ndterms = (int32_t)g.dev_dterms.size();
+    std::vector<int32_t> nf32(g.nfunc.begin(), g.nfunc.end());
+    hipError_t e = upload(c, g.dev_terms, &d.terms, c->gallocs);
+    if (e == hipSuccess) e = upload(c, g.dev_steps, &d.steps, c->gallocs);
+    if (e == hipSuccess) e = upload(c, g.ftab, &d.ftab64, c->gallocs);
+    if (e == hipSuccess) e = upload(c, g.dev_dterms, &d.dterms, c->gallocs);
+    if (e == hipSuccess) e = upload(c, g.omega, &d.omega, c->gallocs);
+    if (e == hipSuccess) e = upload(c, nf32, &d.nfunc, c->gallocs);
+    if (e != hipSuccess) return hip_fail(c, e, "gradient operator upload");
+    if (nv::grad_lds_bytes(d) > 64 * 1024) return fail(c, NV_ERR_ARG, "gradient factor table exceeds the LDS budget");
+    c->gop_ready = true;
+    return NV_OK;
+}
+
+// Maps the accumulated bilinear forms onto the nadvibs.in slots and the
+// frequencies (chain rule through c * prod_k w_k^{-p_k/2}, Source/top.f90:212-217).
+static void map_gradients(nv_ctx *c, const std::vector<double> &gacc, double *g_omega, double *g_concoef,
+                          double *g_pot) {
+    const nv::Operator &op = c->op, &g = c->gop;
+    const int ns = (int)op.ns, nss = ns * ns;
+    const int nT = c->gdev.nterms, nD = c->gdev.ndterms;
+    const bool lower = (op.quirks & NV_QUIRK_LOWERDIAG) != 0;
+    std::vector<int> bi(op.nstblks), bj(op.nstblks);
+    for (int i = 1; i <= ns; ++i)
+        for (int j = 1; j <= i; ++j) { bi[i * (i - 1) / 2 + j - 1] = i - 1; bj[i * (i - 1) / 2 + j - 1] = j - 1; }
+    auto GT = [&](int t, int a, int b) { return gacc[(size_t)t * nss + a * ns + b]; };
+    auto GD = [&](int t, int a, int b) { return gacc[(size_t)(nT + t) * nss + a * ns + b]; };
+    auto GC = [&](int a, int b) { return gacc[(size_t)(nT + nD + op.M) * nss + a * ns + b]; };
+    std::vector<double> gw(op.M, 0.0);
+    for (int64_t k = 0; k < op.M; ++k) gw[k] = gacc[(size_t)(nT + nD + k) * nss];
+    // signature -> gradient-operator term
+    std::map<std::vector<int>, int> sig;
+    for (size_t t = 0; t < g.terms.size(); ++t) {
+        std::vector<int> key;
+        for (int j = 0; j < g.terms[t].cnt; ++j) { key.push_back(g.terms[t].mode[j]); key.push_back(g.terms[t].pow[j]); }
+        sig[key] = (int)t;
+    }
+    for (int64_t p = 1; p <= op.ordr; ++p) {
+        auto mons = nv::monomials(op.M, p);
+        for (size_t k = 0; k < mons.size(); ++k) {
+            int um[NV_MAXF], up[NV_MAXF], cnt = 0;
+            for (int v : mons[k]) {
+                int f = -1;
+                for (int j = 0; j < cnt; ++j) if (um[j] == v) f = j;
+                if (f < 0) { um[cnt] = v; up[cnt] = 1; ++cnt; } else up[f]++;
+            }
+            std::vector<int> key;
+            for (int j = 0; j < cnt; ++j) { key.push_back(um[j]); key.push_back(up[j]); }
+            const int t = sig.count(key) ? sig[key] : -1;
+            const int dt = t >= 0 ? g.term_dev[t] : -1, dd = t >= 0 ? g.term_ddev[t] : -1;
+            const double dval = t >= 0 ? g.terms[t].dval : 0.0;
+            for (int64_t b = 0; b < op.nstblks; ++b) {
+                const int i = bi[b], j = bj[b];
+                double v = 0.0;
+                if (dt >= 0) v += GT(dt, i, j) + (i != j ? GT(dt, j, i) : 0.0);
+                if (dd >= 0) v += GD(dd, i, j) + (i != j && !lower ? GD(dd, j, i) : 0.0);
+                v *= dval;
+                const size_t slot = k + (p - 1) * c->ntmax + b * c->ntmax * op.ordr;
+                if (g_pot) g_pot[slot] = v;
+                const double craw = c->potraw[slot];
+                if (std::fabs(craw) >= c->ztoler)
+                    for (int jj = 0; jj < cnt; ++jj) gw[um[jj]] += -0.5 * up[jj] / op.omega[um[jj]] * craw * v;
+            }
+        }
+    }
+    if (g_concoef)
+        for (int64_t b = 0; b < op.nstblks; ++b) {
+            const int i = bi[b], j = bj[b];
+            g_concoef[b] = i == j ? GC(i, i) : GC(i, j) + (lower ? 0.0 : GC(j, i));
+        }
+    if (g_omega)
+        for (int64_t k = 0; k < op.M; ++k) g_omega[k] = gw[k];
+}
+
+int32_t nv_lanczos_adjoint(nv_ctx *c, int64_t n, const double *abar, const double *bbar, int32_t prec,
+                           double *g_omega, double *g_concoef, double *g_potterms) {
+    if (!c) return NV_ERR_ARG;
+    if (!abar || !bbar) return fail(c, NV_ERR_ARG, "abar/bbar missing");
+    if (prec != NV_PREC_F64 && prec != NV_PREC_F32) return fail(c, NV_ERR_ARG, "adjoint H*v supports F64 and F32");
+    if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
+    if (c->opts.reorthog > 0)
+        return fail(c, NV_ERR_STATE, "the adjoint differentiates the plain recurrence (reorthog = 0)");
+    if (c->op.neworigin) return fail(c, NV_ERR_STATE, "the adjoint supports the default start vector only");
+    if (n < 1 || n > c->last_iter) return fail(c, NV_ERR_ARG, "n must lie in 1..last iteration");
+    const bool archive = c->opts.savevecs != 0;
+    if (!archive && !c->cp) return fail(c, NV_ERR_STATE, "the adjoint needs savevecs or checkpoints");
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    int32_t rc = ensure_gop(c);
+    if (rc != NV_OK) return rc;
+    const int64_t N = c->op.N, niter = c->opts.niter;
+    const int64_t K = archive ? n : c->opts.checkpoint;
+    const int nss = (int)(c->op.ns * c->op.ns);
+    const int groups = nv::grad_groups(c->gdev), chunks = nv::grad_chunks(c->gdev);
+    std::vector<double> beta_h(niter + 3);
+    NV_HIP(c, hipMemcpy(beta_h.data(), c->st.beta, (niter + 3) * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<void *> tmp;
+    auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+        hipError_t er = hipMalloc(p, bytes);
+        if (er == hipSuccess) tmp.push_back(*p);
+        return er;
+    };
+    double *qb[3], *ub = nullptr, *hub = nullptr, *part = nullptr, *gpart = nullptr, *scal = nullptr;
+    double *abar_d = nullptr, *bbar_d = nullptr, *gacc = nullptr, *seg = nullptr, *ta = nullptr, *tb = nullptr;
+    hipError_t e = alloc((size_t)3 * N * 8, (void **)&qb[0]);
+    qb[1] = qb[0] + N;
+    qb[2] = qb[0] + 2 * N;
+    if (e == hipSuccess) e = alloc((size_t)N * 8, (void **)&ub);
+    if (e == hipSuccess) e = alloc((size_t)N * 8, (void **)&hub);
+    if (e == hipSuccess) e = alloc((size_t)5 * nv::adj_blocks(N) * 8, (void **)&part);
+    if (e == hipSuccess) e = alloc((size_t)groups * chunks * nss * 8, (void **)&gpart);
+    if (e == hipSuccess) e = alloc(8 * 8, (void **)&scal);
+    if (e == hipSuccess) e = alloc((size_t)(n + 1) * 8, (void **)&abar_d);
+    if (e == hipSuccess) e = alloc((size_t)(n + 2) * 8, (void **)&bbar_d);
+    if (e == hipSuccess) e = alloc((size_t)groups * nss * 8, (void **)&gacc);
+    if (!archive) {
+        if (e == hipSuccess) e = alloc((size_t)(K + 2) * N * 8, (void **)&seg);
+        if (e == hipSuccess) e = alloc((size_t)(niter + 3) * 8, (void **)&ta);
+        if (e == hipSuccess) e = alloc((size_t)(niter + 3) * 8, (void **)&tb);
+    }
+    auto done = [&](int32_t code) {
+        for (void *p : tmp) hipFree(p);
+        return code;
+    };
+    if (e != hipSuccess) return done(hip_fail(c, e, "adjoint allocation"));
+    hipStream_t s = c->stream;
+    e = hipMemcpyAsync(abar_d, abar, n * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(bbar_d, bbar, (n + 1) * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(qb[0], 0, (size_t)3 * N * 8, s);
+    if (e == hipSuccess) e = hipMemsetAsync(gacc, 0, (size_t)groups * nss * 8, s);
+    int ia = 0, ib = 1, ic = 2;  // qbar_{i+1}, qbar_i, qbar_{i-1}
+    bool have_hub = false;
+    const int64_t nseg = (n + K - 1) / K;
+    for (int64_t sg = nseg - 1; sg >= 0 && e == hipSuccess; --sg) {
+        const int64_t s0 = sg * K + 1, s1 = std::min((sg + 1) * K, n);
+        const double *V = c->st.V;
+        int64_t nslots = c->st.nslots;
+        bool end_unnormalised = s1 == c->last_iter;
+        if (!archive) {
+            // recompute q_{s0-1} .. q_{s1}, r_{s1} from checkpoint sg
+            nv::LzState ts = c->st;
+            ts.V = seg;
+            ts.nslots = K + 2;
+            ts.alpha = ta;
+            ts.beta = tb;
+            e = hipMemcpyAsync(ta, c->st.alpha, (niter + 3) * 8, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(tb, c->st.beta, (niter + 3) * 8, hipMemcpyDeviceToDevice, s);
+            const double *cpk = c->cp + (size_t)sg * 2 * N;
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(seg + ((s0 - 1) % (K + 2)) * N, cpk, N * 8, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(seg + (s0 % (K + 2)) * N, cpk + N, N * 8, hipMemcpyDeviceToDevice, s);
+            for (int64_t j = s0; j <= s1 && e == hipSuccess; ++j) {
+                e = nv::launch_set_iter(c->d_iter, j, 0, s);
+                if (e == hipSuccess) e = nv::launch_normalize(c->dev, c->opts.prec, ts, s);
+                if (e == hipSuccess) e = launch_hv_iter_st(c, ts);
+                if (e == hipSuccess) e = nv::launch_lanczos_step(c->dev, c->opts.prec, ts, s);
+            }
+            if (e != hipSuccess) break;
+            std::vector<double> ra(s1 - s0 + 1), rb(s1 - s0 + 1), oa(s1 - s0 + 1), ob(s1 - s0 + 1);
+            e = hipMemcpyAsync(ra.data(), ta + s0, ra.size() * 8, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(rb.data(), tb + s0, rb.size() * 8, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(oa.data(), c->st.alpha + s0, oa.size() * 8, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(ob.data(), c->st.beta + s0, ob.size() * 8, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) break;
+            if (std::memcmp(ra.data(), oa.data(), ra.size() * 8) != 0 || std::memcmp(rb.data(), ob.data(), rb.size() * 8) != 0)
+                return done(fail(c, NV_ERR_STATE, "checkpoint recomputation differs from the stored run at iterations " +
+                                                      std::to_string(s0) + ".." + std::to_string(s1)));
+            V = seg;
+            nslots = K + 2;
+            end_unnormalised = true;
+        }
+        for (int64_t i = s1; i >= s0 && e == hipSuccess; --i) {
+            const double *qi = V + (i % nslots) * N, *qn = V + ((i + 1) % nslots) * N;
+            const double *qp = i > 1 ? V + ((i - 1) % nslots) * N : nullptr;
+            const double sn = (i == s1 && end_unnormalised) ? 1.0 / beta_h[i] : 1.0;
+            e = nv::launch_adj_step(N, i, qb[ia], have_hub ? hub : nullptr, qn, sn, qi, qp, c->st.alpha, c->st.beta,
+                                    abar_d, bbar_d, part, scal, ub, qb[ib], qb[ic], s);
+            if (e == hipSuccess) e = hv_apply(c, prec, ub, hub, s);
+            if (e == hipSuccess) e = nv::launch_grad_contract(c->gdev, ub, qi, gpart, gacc, s);
+            if (e == hipSuccess) e = nv::launch_adj_zero(N, qb[ia], s);
+            have_hub = true;
+            const int t = ia;
+            ia = ib; ib = ic; ic = t;
+        }
+    }
+    std::vector<double> gh((size_t)groups * nss);
+    if (e == hipSuccess) e = hipMemcpyAsync(gh.data(), gacc, gh.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return done(hip_fail(c, e, "adjoint"));
+    map_gradients(c, gh, g_omega, g_concoef, g_potterms);
+    return done(NV_OK);
+}
+
+static int32_t spectrum_common(nv_ctx *c, int64_t n, const double *alpha, const double *beta, int64_t ng,
+                               const double *grid, double sigma, int32_t origin, double e0, const double *gbar,
+                               double *env, double *abar, double *bbar) {
+    if (!c) return NV_ERR_ARG;
+    if (n < 1 || ng < 1 || !alpha || !beta || !grid || !(sigma > 0) || (origin != 0 && origin != 1))
+        return fail(c, NV_ERR_ARG, "bad spectrum arguments");
+    std::vector<double> root;
+    double *vec = nullptr;
+    int32_t rc = tridiag_solve(c, n, alpha, beta, root, &vec);
+    if (rc != NV_OK) return rc;
+    std::vector<void *> tmp{vec};
+    auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+        hipError_t er = hipMalloc(p, bytes);
+        if (er == hipSuccess) tmp.push_back(*p);
+        return er;
+    };
+    const double ref = origin == 0 ? root[0] : e0;
+    const double cw = 219474.6313705;  // AU2WAVE (Source/progdata.f90:7)
+    double *th = nullptr, *gr = nullptr, *gb = nullptr, *out = nullptr, *work = nullptr, *ab = nullptr, *bb = nullptr;
+    hipStream_t s = c->stream;
+    hipError_t e = alloc(n * 8, (void **)&th);
+    if (e == hipSuccess) e = alloc(ng * 8, (void **)&gr);
+    if (e == hipSuccess) e = hipMemcpyAsync(th, root.data(), n * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(gr, grid, ng * 8, hipMemcpyHostToDevice, s);
+    if (env) {
+        if (e == hipSuccess) e = alloc(ng * 8, (void **)&out);
+        if (e == hipSuccess) e = nv::launch_spec_env(n, th, vec, ref, cw, sigma, ng, gr, out, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(env, out, ng * 8, hipMemcpyDeviceToHost, s);
+    } else {
+        if (e == hipSuccess) e = alloc(ng * 8, (void **)&gb);
+        if (e == hipSuccess) e = alloc(((size_t)2 * n + 8 + (size_t)2 * n * n) * 8, (void **)&work);
+        if (e == hipSuccess) e = alloc(n * 8, (void **)&ab);
+        if (e == hipSuccess) e = alloc((n + 1) * 8, (void **)&bb);
+        if (e == hipSuccess) e = hipMemsetAsync(bb, 0, (n + 1) * 8, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(gb, gbar, ng * 8, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = nv::launch_spec_adjoint(n, th, vec, ref, origin == 0, cw, sigma, ng, gr, gb, work, ab, bb, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(abar, ab, n * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(bbar, bb, (n + 1) * 8, hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void *p : tmp) hipFree(p);
+    if (e != hipSuccess) return hip_fail(c, e, "spectrum");
+    return NV_OK;
+}
+
+int32_t nv_spectrum(nv_ctx *c, int64_t n, const double *alpha, const double *beta, int64_t ng, const double *grid,
+                    double sigma, int32_t origin, double e0, double *env) {
+    if (!env) return fail(c, NV_ERR_ARG, "env is NULL");
+    return spectrum_common(c, n, alpha, beta, ng, grid, sigma, origin, e0, nullptr, env, nullptr, nullptr);
+}
+
+int32_t nv_spectrum_adjoint(nv_ctx *c, int64_t n, const double *alpha, const double *beta, int64_t ng,
+                            const double *grid, double sigma, int32_t origin, double e0, const double *gbar,
+                            double *abar, double *bbar) {
+    if (!gbar || !abar || !bbar) return fail(c, NV_ERR_ARG, "gbar/abar/bbar missing");
+    return spectrum_common(c, n, alpha, beta, ng, grid, sigma, origin, e0, gbar, nullptr, abar, bbar);
+}
+
 // ---- Gaussian broadening ----------------------------------------------------
 int32_t nv_convolve(nv_ctx *c, int64_t n, const double *x, const double *y, int64_t ng, const double *grid,
                     double sigma, double *env) {
@@ -853,8 +1189,8 @@ int32_t nv_load_vectors(nv_ctx *c, const char *path, int64_t iter, const double 
     if (!c || !path || !alpha || !beta) return NV_ERR_ARG;
     if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
     if (iter < 1 || iter >= c->opts.niter) return fail(c, NV_ERR_ARG, "restart iteration out of range");
-    if (c->opts.reorthog > 0 || c->opts.savevecs != 0)
-        return fail(c, NV_ERR_STATE, "a restart cannot rebuild the archive of earlier Lanczos vectors");
+    if (c->opts.reorthog > 0 || c->opts.savevecs != 0 || c->cp)
+        return fail(c, NV_ERR_STATE, "a restart cannot rebuild the archive or checkpoints of earlier Lanczos vectors");
     (void)omega;
     const int64_t N = c->op.N;
     std::vector<double> buf(3 * N);

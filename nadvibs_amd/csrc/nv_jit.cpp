@@ -85,6 +85,11 @@ std::string jit_source(const Operator &op, int prec) {
     s << "};\n";
     // helpers
     s << R"(
+// workgroup b runs on XCD b % 8: give each XCD one contiguous range of rows
+__device__ __forceinline__ int nv_xcd_block(int b, int g) {
+    const int x = b & 7, j = b >> 3, q = g >> 3, rem = g & 7;
+    return x * q + (x < rem ? x : rem) + j;
+}
 __device__ __forceinline__ double nv_block_sum(double v) {
     __shared__ double red[4];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -104,7 +109,7 @@ __device__ __forceinline__ double nv_block_sum(double v) {
     b << "    __shared__ T tab[" << (tab.empty() ? 1 : tab.size()) << "];\n";
     b << "    for (int i = threadIdx.x; i < " << tab.size() << "; i += 256) tab[i] = NV_TAB[i];\n";
     b << "    __syncthreads();\n";
-    b << "    const int r = blockIdx.x * 256 + threadIdx.x;\n";
+    b << "    const int r = nv_xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;\n";
     b << "    const bool live = r < NV_D;\n";
     b << "    unsigned t = live ? (unsigned)r : 0u;\n";
     for (int64_t k = op.M - 1; k >= 0; --k) {

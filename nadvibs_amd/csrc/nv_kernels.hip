@@ -6,8 +6,9 @@
 //                  table sit in LDS; coupling matrices are read as uniform
 //                  (scalar) loads.
 //  * normalize / lanczos_step   the three-term recurrence in FP64 with the
-//                  dots reduced per block and finished by the last block to
-//                  arrive (fixed summation order: reproducible run to run).
+//                  dots reduced to per-block partials and summed by a
+//                  one-block finish kernel (fixed summation order:
+//                  reproducible run to run).
 //  * tri_*         Sturm bisection and inverse iteration for the tridiagonal
 //                  eigenproblem, one root per thread.
 #include <hip/hip_runtime.h>
@@ -16,6 +17,15 @@
 #include "nv_kernels.hpp"
 
 namespace nv {
+
+// Workgroups of one launch go to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8).  H*v gathers x at +-stride of every mode, so a workgroup reads the
+// rows of its neighbours along the slow modes; mapping each XCD onto one
+// contiguous range of rows keeps that window inside the XCD's own L2.
+__device__ __forceinline__ int xcd_block(int b, int g) {
+    const int x = b & 7, j = b >> 3, q = g >> 3, rem = g & 7;
+    return x * q + (x < rem ? x : rem) + j;
+}
 
 // ------------------------------------------------------------------------
 // reductions
@@ -213,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void hv_full_kernel(NvOpDev op, const doubl
     uint8_t *dig;
     const TAcc *ftab = stage_tables<TAcc>(op, dig);
     const int tid = threadIdx.x;
-    const int r = blockIdx.x * kBlock + tid;
+    const int r = xcd_block(blockIdx.x, gridDim.x) * kBlock + tid;
     const bool live = r < op.D;
     if (live) digits_energy(op, r, dig, tid);
     __syncthreads();
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void hv_iter_kernel(NvOpDev op, LzState st)
     uint8_t *dig;
     const TAcc *ftab = stage_tables<TAcc>(op, dig);
     const int tid = threadIdx.x;
-    const int r = blockIdx.x * kBlock + tid;
+    const int r = xcd_block(blockIdx.x, gridDim.x) * kBlock + tid;
     const bool live = r < op.D;
     const int64_t it = *st.iter;
     const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;

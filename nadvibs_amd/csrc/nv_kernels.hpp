@@ -89,4 +89,28 @@ void gram_shape(int64_t D, int R, int &npairs, int &nchunks, int64_t &rows_per_c
 // G (R x R) = A^T A, column c of A at A + c*D.
 hipError_t launch_gram(const double *A, int64_t D, int R, double *part, double *G, hipStream_t s);
 
+// ---- adjoint (nv_adjoint.hip) ----
+int adj_blocks(int64_t N);
+// One reverse step of the recurrence at iteration i (see nv_adjoint.hip).
+// part holds 5 * adj_blocks(N) doubles, scal 3.
+hipError_t launch_adj_step(int64_t N, int64_t i, double *qbn, const double *hub, const double *qn, double sn,
+                           const double *qi, const double *qp, const double *alpha, const double *beta,
+                           const double *abar, double *bbar, double *part, double *scal, double *ub, double *qbi,
+                           double *qbp, hipStream_t s);
+hipError_t launch_adj_zero(int64_t N, double *v, hipStream_t s);
+// Bilinear forms ubar.(P_t q) of every monomial of g, added to gacc
+// [grad_groups][ns*ns]; part holds grad_groups * grad_chunks * ns*ns doubles.
+int grad_groups(const NvOpDev &g);
+int grad_chunks(const NvOpDev &g);
+size_t grad_lds_bytes(const NvOpDev &g);
+hipError_t launch_grad_contract(const NvOpDev &g, const double *ub, const double *q, double *part, double *gacc,
+                                hipStream_t s);
+// Broadened spectrum of tridiagonal eigenpairs (theta on the device, vec as
+// TriWork::vec) and its adjoint; work holds 2n + 8 + 2n^2 doubles.
+hipError_t launch_spec_env(int64_t n, const double *theta, const double *vec, double ref, double c, double sigma,
+                           int64_t ng, const double *grid, double *env, hipStream_t s);
+hipError_t launch_spec_adjoint(int64_t n, const double *theta, const double *vec, double ref, int lowest, double c,
+                               double sigma, int64_t ng, const double *grid, const double *gbar, double *work,
+                               double *abar, double *bbar, hipStream_t s);
+
 }  // namespace nv

@@ -199,7 +199,10 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
     }
     // coupling matrices, steps and the term-count check
     int64_t trm_rows = 0;
-    for (auto &t : op.terms) {
+    op.term_dev.assign(op.terms.size(), -1);
+    op.term_ddev.assign(op.terms.size(), -1);
+    for (size_t ti = 0; ti < op.terms.size(); ++ti) {
+        Term &t = op.terms[ti];
         std::vector<double> C(ns * ns, 0.0);
         int npairs = 0;
         for (size_t q = 0; q < t.blocks.size(); ++q) {
@@ -246,6 +249,7 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
             }
         }
         if (dt.nsteps > 0) {
+            op.term_dev[ti] = (int32_t)op.dev_terms.size();
             op.dev_terms.push_back(dt);
             op.cmat.insert(op.cmat.end(), C.begin(), C.end());
         }
@@ -254,6 +258,7 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
             dd.nf = t.cnt;
             for (int j = 0; j < NV_MAXF; ++j) { dd.mode[j] = 0; dd.code[j] = 0; }
             for (int j = 0; j < t.cnt; ++j) { dd.mode[j] = t.mode[j]; dd.code[j] = code_of(t.mode[j], t.pow[j], 0); }
+            op.term_ddev[ti] = (int32_t)op.dev_dterms.size();
             op.dev_dterms.push_back(dd);
             std::vector<double> Dm(ns * ns, 0.0);
             for (int s = 0; s < ns; ++s)

@@ -46,6 +46,8 @@ struct Operator {
     std::vector<double> dconst;          // [ns][ns]
     int32_t ncodes = 0;
     int64_t nnz_fma = 0;                 // FMAs of one H*v (nonzero structure)
+    std::vector<int32_t> term_dev;       // [terms] index into dev_terms, -1 if the term has no Delta != 0 step
+    std::vector<int32_t> term_ddev;      // [terms] index into dev_dterms, -1 if it has no Delta = 0 part
 
     // start vector data
     int32_t neworigin = 0;

@@ -107,9 +107,10 @@ class Engine:
 
     # -- Lanczos ---------------------------------------------------------------
     def lanczos_init(self, niter, reorthog=0, prec="f64", chkorthog=100, seed=12345, noise=1.0, pro_eps=0.0,
-                     savevecs=False):
+                     savevecs=False, checkpoint=0):
         o = _lib.NvLanczosOpts(niter=niter, reorthog=reorthog, prec=_lib.PREC[prec], chkorthog=chkorthog,
-                               seed=seed, noise_scale=noise, pro_eps=pro_eps, savevecs=1 if savevecs else 0)
+                               seed=seed, noise_scale=noise, pro_eps=pro_eps, savevecs=1 if savevecs else 0,
+                               checkpoint=checkpoint)
         self._check(self.lib.nv_lanczos_init(self.h, ctypes.byref(o)))
         self.niter = niter
         self.reorthog = reorthog
@@ -193,6 +194,43 @@ class Engine:
         if vectors:
             out["vectors"] = vec
         return out
+
+    # -- derivatives -----------------------------------------------------------
+    def select_hv_kernel(self, generic=True):
+        """generic=True skips the run-time compile of the specialised H*v."""
+        self._check(self.lib.nv_select_hv_kernel(self.h, 1 if generic else 0))
+
+    def lanczos_adjoint(self, abar, bbar, prec="f64"):
+        """dL/d(aomega, concoef, potterms) of the last run, given
+        abar = dL/dalpha(1:n) and bbar = dL/dbeta(0:n)."""
+        abar = np.ascontiguousarray(abar, dtype=np.float64)
+        bbar = np.ascontiguousarray(bbar, dtype=np.float64)
+        n = abar.size
+        if bbar.size != n + 1:
+            raise ValueError("bbar must hold beta(0:n)")
+        p = self.prob
+        go, gc = np.zeros(p.nmodes), np.zeros(p.nstblks)
+        gp = np.zeros(np.shape(p.potterms))
+        self._check(self.lib.nv_lanczos_adjoint(self.h, n, dptr(abar), dptr(bbar), _lib.PREC[prec], dptr(go),
+                                                dptr(gc), dptr(gp)))
+        return dict(omega=go, concoef=gc, potterms=gp)
+
+    def spectrum(self, alpha, beta, grid, sigma, origin="lowest", e0=0.0):
+        """Broadened sticks of T(alpha, beta) on grid (cm^-1), origin theta_1
+        ("lowest") or e0 hartree ("fixed")."""
+        alpha, beta, grid = (np.ascontiguousarray(v, dtype=np.float64) for v in (alpha, beta, grid))
+        env = np.empty_like(grid)
+        self._check(self.lib.nv_spectrum(self.h, alpha.size, dptr(alpha), dptr(beta), grid.size, dptr(grid),
+                                         float(sigma), 0 if origin == "lowest" else 1, float(e0), dptr(env)))
+        return env
+
+    def spectrum_adjoint(self, alpha, beta, grid, sigma, gbar, origin="lowest", e0=0.0):
+        alpha, beta, grid, gbar = (np.ascontiguousarray(v, dtype=np.float64) for v in (alpha, beta, grid, gbar))
+        ab, bb = np.empty(alpha.size), np.empty(alpha.size + 1)
+        self._check(self.lib.nv_spectrum_adjoint(self.h, alpha.size, dptr(alpha), dptr(beta), grid.size, dptr(grid),
+                                                 float(sigma), 0 if origin == "lowest" else 1, float(e0), dptr(gbar),
+                                                 dptr(ab), dptr(bb)))
+        return ab, bb
 
     def hv_kernel(self, prec="f64"):
         """Which H*v kernel serves `prec` (compiles it on first use)."""

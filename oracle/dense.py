@@ -49,11 +49,12 @@ def monomial_operator(nfunc, pows, quirks):
     return out
 
 
-def dense_hamiltonian(prob):
-    """H as an (N, N) array in the state-major layout x[s][n_1..n_M]."""
+def dense_hamiltonian(prob, dtype=float):
+    """H as an (N, N) array in the state-major layout x[s][n_1..n_M].
+    dtype=complex accepts complex frequencies (for complex-step derivatives)."""
     ns, nm, nf = prob.nstates, prob.nmodes, list(prob.nfunc)
     D = int(np.prod(nf))
-    H = np.zeros((ns * D, ns * D))
+    H = np.zeros((ns * D, ns * D), dtype=dtype)
     # harmonic part
     grids = np.meshgrid(*[np.arange(v) for v in nf], indexing="ij")
     E = sum((g.reshape(-1) + 0.5) * w for g, w in zip(grids, prob.aomega))

@@ -1,0 +1,97 @@
+"""Adjoint of the Lanczos recurrence and of the broadened spectrum.
+
+CPU: the oracle's reverse sweep (oracle/adjoint.py) against complex-step
+derivatives of the recurrence, and its spectrum adjoint against central
+differences (including a near-degenerate "ghost" pair, where the divided
+differences must stay finite).
+"""
+import numpy as np
+import pytest
+
+from nadvibs_amd.synth import make_problem
+from oracle.adjoint import (complex_step_lanczos, lanczos_adjoint, lanczos_basis, parameter_matrices, spectrum,
+                            spectrum_adjoint)
+from oracle.dense import dense_hamiltonian
+
+
+def _prob(ns=2, nf=(4, 3, 3), ordr=2, offe=0.0, seed=21, quirks=3):
+    return make_problem(ns, list(nf), ordr, (600, 2800), 250.0, 60.0, 40.0, cubic=15.0 if ordr > 2 else 0.0,
+                        seed=seed, quirks=quirks, offdiag_even=offe)
+
+
+def _start(p):
+    v0 = np.zeros(p.n)
+    v0[np.arange(p.nstates) * p.dimen] = 1.0
+    return v0
+
+
+@pytest.mark.parametrize("case", [dict(), dict(ns=3, nf=(3, 3, 2), ordr=3, offe=20.0),
+                                  dict(ns=3, nf=(3, 3, 2), ordr=3, offe=20.0, quirks=0)],
+                         ids=["ns2-ordr2", "ns3-ordr3-q2", "ns3-exact"])
+def test_oracle_lanczos_adjoint_matches_complex_step(case):
+    p = _prob(**case)
+    H = dense_hamiltonian(p)
+    v0 = _start(p)
+    n = 10
+    a, b, Q = lanczos_basis(H, v0, n)
+    rng = np.random.default_rng(3)
+    abar, bbar = rng.standard_normal(n), rng.standard_normal(n + 1)
+    dHs = [m for m in parameter_matrices(p) if m is not None]
+    g = lanczos_adjoint(H, Q, a, b, abar, bbar, dHs)
+    ref = np.array([abar @ da + bbar @ db for da, db in (complex_step_lanczos(H, dH, v0, n) for dH in dHs)])
+    assert np.max(np.abs(g - ref)) <= 1e-9 * np.max(np.abs(ref))
+
+
+def test_parameter_matrices_rebuild_h():
+    """H is affine in the coefficients: sum c * dH/dc + H(c = 0) gives H back."""
+    p = _prob(ns=3, nf=(3, 3, 2), ordr=3, offe=20.0, quirks=0)
+    mats = parameter_matrices(p)
+    M = p.nmodes
+    from oracle.adjoint import _zero_problem
+    H = dense_hamiltonian(_zero_problem(p))
+    coefs = list(p.concoef) + list(p.potterms.reshape(-1))
+    for c, m in zip(coefs, mats[M:]):
+        if m is not None:
+            H = H + c * m
+    np.testing.assert_allclose(H, dense_hamiltonian(p), atol=1e-15)
+
+
+def _tri(n, seed=4):
+    p = _prob(nf=(5, 4, 4), seed=seed)
+    H = dense_hamiltonian(p)
+    a, b, _ = lanczos_basis(H, _start(p), n)
+    return a, b
+
+
+@pytest.mark.parametrize("origin", ["lowest", "fixed"])
+def test_spectrum_adjoint_matches_central_differences(origin):
+    a, b = _tri(30)
+    grid = np.linspace(-200.0, 9000.0, 300)
+    sigma = 40.0
+    e0 = a.min() - 0.001
+    gbar = np.random.default_rng(5).standard_normal(grid.size)
+    ab, bb = spectrum_adjoint(a, b, grid, sigma, gbar, origin, e0)
+    L = lambda a_, b_: gbar @ spectrum(a_, b_, grid, sigma, origin, e0)
+    h = 1e-9
+    fa = np.array([(L(a + h * e, b) - L(a - h * e, b)) / (2 * h) for e in np.eye(a.size)])
+    fb = np.array([(L(a, b + h * e) - L(a, b - h * e)) / (2 * h) for e in np.eye(b.size)])
+    scale = max(np.max(np.abs(fa)), np.max(np.abs(fb)))
+    assert np.max(np.abs(ab - fa)) <= 1e-5 * scale
+    assert np.max(np.abs(bb - fb)) <= 1e-5 * scale
+    assert bb[0] == 0.0 and bb[-1] == 0.0
+
+
+def test_spectrum_adjoint_finite_on_ghost_pair():
+    """Two decoupled copies of one Ritz value (a ghost): the divided
+    differences of the pair fall back to the derivative and the gradient
+    matches central differences."""
+    a = np.array([0.010, 0.012, 0.012, 0.015])
+    b = np.array([1.0, 1e-3, 1e-14, 2e-3, 0.0])
+    grid = np.linspace(-100.0, 2000.0, 200)
+    gbar = np.random.default_rng(6).standard_normal(grid.size)
+    ab, bb = spectrum_adjoint(a, b, grid, 25.0, gbar, "fixed", 0.009)
+    assert np.all(np.isfinite(ab)) and np.all(np.isfinite(bb))
+    L = lambda a_, b_: gbar @ spectrum(a_, b_, grid, 25.0, "fixed", 0.009)
+    h = 1e-10
+    fa = np.array([(L(a + h * e, b) - L(a - h * e, b)) / (2 * h) for e in np.eye(a.size)])
+    assert np.max(np.abs(ab - fa)) <= 1e-4 * np.max(np.abs(fa))

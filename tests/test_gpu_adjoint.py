@@ -1,0 +1,152 @@
+"""GPU: the library's reverse sweep and spectrum adjoint against the oracle
+(oracle/adjoint.py, itself pinned by complex-step derivatives), checkpoint
+recomputation against the kept basis, mixed precision against FP64, and the
+torch binding end to end against central differences.
+
+Tolerances: FP64 adjoint vs oracle 1e-9 of the largest gradient component
+(the forward runs differ by rounding only); checkpointed vs kept basis
+exactly equal (the recomputation is checked bit for bit); FP32 H*v vs FP64
+1e-5 of the largest component; central differences 1e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+from nadvibs_amd.adjoint import broadened_spectrum, lanczos_coefficients
+from nadvibs_amd.engine import Engine
+from nadvibs_amd.synth import make_problem
+from oracle.adjoint import lanczos_adjoint, lanczos_basis, parameter_matrices, spectrum, spectrum_adjoint
+from oracle.dense import dense_hamiltonian
+
+pytestmark = pytest.mark.gpu
+
+
+def _prob(ns=2, nf=(5, 4, 4), ordr=2, offe=0.0, seed=31, quirks=3):
+    return make_problem(ns, list(nf), ordr, (600, 2800), 250.0, 60.0, 40.0, cubic=15.0 if ordr > 2 else 0.0,
+                        seed=seed, quirks=quirks, offdiag_even=offe)
+
+
+def _flat(g, p):
+    """Gradient dict in the oracle's parameter order (invalid POTterms slots dropped)."""
+    from math import comb
+    out = list(g["omega"]) + list(g["concoef"])
+    for idx in np.ndindex(*p.potterms.shape):
+        if idx[2] < comb(p.nmodes + idx[1], idx[1] + 1):
+            out.append(g["potterms"][idx])
+    return np.array(out)
+
+
+def _gpu_grad(p, n, abar, bbar, prec="f64", checkpoint=0, generic=False):
+    with Engine(p) as eng:
+        if generic:
+            eng.select_hv_kernel(True)
+        eng.lanczos_init(n, prec=prec, savevecs=checkpoint == 0, checkpoint=checkpoint)
+        eng.run(1, n)
+        a, b, _ = eng.coefficients()
+        start = eng.start_vector()
+        g = eng.lanczos_adjoint(abar, bbar, prec)
+    return a, b, start, g
+
+
+@pytest.mark.parametrize("case", [dict(), dict(ns=3, nf=(3, 4, 3), ordr=3, offe=20.0),
+                                  dict(ns=3, nf=(3, 4, 3), ordr=3, offe=20.0, quirks=0)],
+                         ids=["ns2-ordr2", "ns3-ordr3-q2", "ns3-exact"])
+def test_adjoint_matches_oracle(case):
+    p = _prob(**case)
+    n = 14
+    rng = np.random.default_rng(7)
+    abar, bbar = rng.standard_normal(n), rng.standard_normal(n + 1)
+    a, b, start, g = _gpu_grad(p, n, abar, bbar)
+    H = dense_hamiltonian(p)
+    a_o, b_o, Q = lanczos_basis(H, start, n)
+    np.testing.assert_allclose(a, a_o, rtol=1e-11, atol=1e-15)
+    mats = [m for m in parameter_matrices(p) if m is not None]
+    ref = lanczos_adjoint(H, Q, a_o, b_o, abar, bbar, mats)
+    got = _flat(g, p)
+    assert got.size == ref.size
+    assert np.max(np.abs(got - ref)) <= 1e-9 * np.max(np.abs(ref))
+
+
+def test_adjoint_checkpoints_equal_kept_basis():
+    p = _prob(ns=3, nf=(8, 7, 6, 5), seed=33)
+    n = 40
+    rng = np.random.default_rng(8)
+    abar, bbar = rng.standard_normal(n), rng.standard_normal(n + 1)
+    _, _, _, g_keep = _gpu_grad(p, n, abar, bbar)
+    for K in (7, 40, 64):
+        _, _, _, g_cp = _gpu_grad(p, n, abar, bbar, checkpoint=K)
+        for key in g_keep:
+            np.testing.assert_array_equal(g_cp[key], g_keep[key])
+
+
+def test_adjoint_mixed_precision_within_1e5_of_fp64():
+    p = _prob(ns=3, nf=(8, 7, 6, 5), seed=34)
+    n = 30
+    rng = np.random.default_rng(9)
+    abar, bbar = rng.standard_normal(n), rng.standard_normal(n + 1)
+    _, _, _, g64 = _gpu_grad(p, n, abar, bbar, "f64")
+    _, _, _, g32 = _gpu_grad(p, n, abar, bbar, "f32")
+    f64, f32 = _flat(g64, p), _flat(g32, p)
+    assert np.max(np.abs(f32 - f64)) <= 1e-5 * np.max(np.abs(f64))
+
+
+@pytest.mark.parametrize("origin", ["lowest", "fixed"])
+def test_spectrum_and_adjoint_match_oracle(origin):
+    p = _prob()
+    H = dense_hamiltonian(p)
+    v0 = np.zeros(p.n)
+    v0[np.arange(p.nstates) * p.dimen] = 1.0
+    a, b, _ = lanczos_basis(H, v0, 40)
+    grid = np.linspace(-300.0, 9000.0, 1000)
+    e0 = a.min() - 0.002
+    gbar = np.random.default_rng(10).standard_normal(grid.size)
+    with Engine(p) as eng:
+        env = eng.spectrum(a, b, grid, 30.0, origin, e0)
+        ab, bb = eng.spectrum_adjoint(a, b, grid, 30.0, gbar, origin, e0)
+    ref = spectrum(a, b, grid, 30.0, origin, e0)
+    assert np.max(np.abs(env - ref)) <= 1e-10 * np.max(np.abs(ref))
+    ra, rb = spectrum_adjoint(a, b, grid, 30.0, gbar, origin, e0)
+    scale = max(np.max(np.abs(ra)), np.max(np.abs(rb)))
+    assert np.max(np.abs(ab - ra)) <= 1e-8 * scale
+    assert np.max(np.abs(bb - rb)) <= 1e-8 * scale
+
+
+def test_torch_loss_gradient_matches_central_differences():
+    """loss = |env - target|_2 through both autograd functions; a few
+    coefficients, a concoef and every frequency by central differences of
+    the device forward."""
+    p = _prob(seed=35)
+    n = 24
+    grid = torch.linspace(-200.0, 8000.0, 600, dtype=torch.float64)
+    sigma = 60.0
+    rng = np.random.default_rng(11)
+    target = torch.from_numpy(np.abs(rng.standard_normal(grid.numel())) * 0.05)
+    omega = torch.tensor(p.aomega, dtype=torch.float64, requires_grad=True)
+    concoef = torch.tensor(p.concoef, dtype=torch.float64, requires_grad=True)
+    potterms = torch.tensor(p.potterms, dtype=torch.float64, requires_grad=True)
+    dev = Engine(p)
+
+    def loss_of(w, c, pt):
+        a, b = lanczos_coefficients(p, w, c, pt, n)
+        env = broadened_spectrum(a, b, grid, sigma, "fixed", float(p.concoef[0]), engine=dev)
+        return torch.linalg.vector_norm(env - target)
+
+    L = loss_of(omega, concoef, potterms)
+    L.backward()
+    checks = [("omega", k) for k in range(p.nmodes)] + [("concoef", 1)]
+    nz = np.argwhere(np.abs(p.potterms) > 0)
+    checks += [("potterms", tuple(nz[i])) for i in rng.choice(len(nz), 6, replace=False)]
+    vals = {"omega": omega, "concoef": concoef, "potterms": potterms}
+    for name, idx in checks:
+        x = vals[name].detach()
+        h = 1e-6 * max(abs(float(x[idx])), 1e-4)
+        up, dn = [vals[k].detach().clone() for k in ("omega", "concoef", "potterms")], \
+                 [vals[k].detach().clone() for k in ("omega", "concoef", "potterms")]
+        j = ("omega", "concoef", "potterms").index(name)
+        up[j][idx] += h
+        dn[j][idx] -= h
+        with torch.no_grad():
+            fd = (float(loss_of(*up)) - float(loss_of(*dn))) / (2 * h)
+        got = float(vals[name].grad[idx])
+        assert abs(got - fd) <= 1e-5 * max(abs(fd), 1e-3 * abs(float(L))), (name, idx, got, fd)
+    dev.close()
