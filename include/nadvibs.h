@@ -207,7 +207,11 @@ int32_t nv_identify_roots(nv_ctx *ctx, int64_t n, const double *alpha,
 /* ---- derivatives (no counterpart in the reference: gradients of a
  *      spectrum loss for fitting the coupling constants) ------------------ */
 /* Reverse sweep of the recurrence of the last nv_lanczos_run /
- * per-iteration calls (reorthog = 0, default start vector): given
+ * per-iteration calls (reorthog = 0, default start vector): the exact
+ * derivative of the computed coefficients.  The backward recurrence amplifies
+ * rounding along converged Ritz directions (about 2x per iteration), so it is
+ * accurate for short runs only (n up to ~30 in FP64); spectral losses use
+ * nv_krylov_gradient.  Given
  * abar = dL/dalpha(1:n) and bbar = dL/dbeta(0:n), accumulates
  *   g_omega[nmodes]      dL/daomega (hartree),
  *   g_concoef[nstblks]   dL/dconcoef,
@@ -228,14 +232,27 @@ int32_t nv_lanczos_adjoint(nv_ctx *ctx, int64_t n, const double *abar,
 int32_t nv_spectrum(nv_ctx *ctx, int64_t n, const double *alpha,
                     const double *beta, int64_t ng, const double *grid,
                     double sigma, int32_t origin, double e0, double *env);
-/* Given gbar = dL/denv: abar = dL/dalpha(1:n), bbar = dL/dbeta(0:n) (bbar[0]
- * = bbar[n] = 0), by divided differences of the eigenvalues (finite for
- * near-degenerate Ritz values). */
+/* Given gbar = dL/denv: tbar = dL/dT as a full symmetric n x n matrix
+ * (S (D o a a^T) S^T, D the divided differences of the eigenvalues: finite
+ * for near-degenerate Ritz values), and its tridiagonal part abar =
+ * dL/dalpha(1:n), bbar = dL/dbeta(0:n) (bbar[0] = bbar[n] = 0).  Outputs may
+ * be NULL. */
 int32_t nv_spectrum_adjoint(nv_ctx *ctx, int64_t n, const double *alpha,
                             const double *beta, int64_t ng,
                             const double *grid, double sigma, int32_t origin,
                             double e0, const double *gbar, double *abar,
-                            double *bbar);
+                            double *bbar, double *tbar);
+/* The gradient of a spectral loss L(T) (tbar = dL/dT from
+ * nv_spectrum_adjoint) with respect to the parameters, projected on the
+ * kept Krylov basis: dL/dtheta = sum_jk tbar[j][k] q_j . (dH/dtheta) q_k.
+ * It is the derivative of v0^T f(H) v0 in the Krylov space; it agrees with
+ * central differences of the Lanczos-computed loss once the spectrum in the
+ * window has converged, and it is numerically stable for any n (unlike the
+ * reverse sweep of nv_lanczos_adjoint).  Needs savevecs or reorthog > 0.
+ * Output layouts as nv_lanczos_adjoint.  Synchronises. */
+int32_t nv_krylov_gradient(nv_ctx *ctx, int64_t n, const double *tbar,
+                           double *g_omega, double *g_concoef,
+                           double *g_potterms);
 /* generic != 0: use the generic H*v kernel instead of compiling an
  * operator-specialised one (saves the run-time compile for operators used
  * once, e.g. inside a fit).  Call before nv_lanczos_init. */

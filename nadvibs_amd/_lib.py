@@ -72,7 +72,8 @@ SIGNATURES = [
     ("nv_spectrum", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp, ctypes.c_double,
                                      ctypes.c_int32, ctypes.c_double, _dp]),
     ("nv_spectrum_adjoint", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp,
-                                             ctypes.c_double, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp]),
+                                             ctypes.c_double, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp, _dp]),
+    ("nv_krylov_gradient", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, _dp]),
     ("nv_select_hv_kernel", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     ("nv_identify_roots", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
                                            _dp, _dp, _i32p, _i64p, _i32p, _i32p, _dp, _dp, _dp, _dp]),

@@ -1,17 +1,22 @@
 """PyTorch autograd over the library's Lanczos run and broadened spectrum.
 
-    alpha, beta = lanczos_coefficients(prob, omega, concoef, potterms, niter)
-    env = broadened_spectrum(alpha, beta, grid, sigma)
+    env = spectrum_of_parameters(prob, omega, concoef, potterms, niter, grid, sigma)
     loss = torch.linalg.vector_norm(env - target)
     loss.backward()          # d loss / d (omega, concoef, potterms)
 
+spectrum_of_parameters is the path for fits: its backward pass takes
+dL/dT from nv_spectrum_adjoint and projects it on the kept Krylov basis
+(nv_krylov_gradient), which is stable for any number of iterations.
+
+lanczos_coefficients exposes (alpha, beta) themselves; its backward pass is
+the exact reverse sweep of the recurrence (nv_lanczos_adjoint, kept basis or
+checkpoints), accurate for short runs only (the sweep amplifies rounding
+along converged Ritz directions).
+
 The forward pass runs the FP64 recurrence on the device (H*v in FP64 or
-FP32); the backward pass is the library's hand-written reverse sweep
-(nv_lanczos_adjoint), which reads the kept Krylov basis or recomputes it
-from checkpoints, and nv_spectrum_adjoint for the spectrum.  Tensors cross
-the boundary as host float64 arrays: the device work stays inside the
-library.  Parameters are the nadvibs.in quantities: aomega (hartree),
-concoef[nstblks] and POTterms [nstblks, ordr, noterms(ordr)].
+FP32).  Tensors cross the boundary as host float64 arrays: the device work
+stays inside the library.  Parameters are the nadvibs.in quantities: aomega
+(hartree), concoef[nstblks] and POTterms [nstblks, ordr, noterms(ordr)].
 """
 import dataclasses
 
@@ -82,3 +87,40 @@ def broadened_spectrum(alpha, beta, grid, sigma, origin="lowest", e0=0.0, engine
     if engine is None:
         raise ValueError("broadened_spectrum needs an Engine for its device")
     return BroadenedSpectrum.apply(alpha, beta, grid, sigma, origin, e0, engine)
+
+
+class SpectrumOfParameters(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, omega, concoef, potterms, prob, niter, grid, sigma, origin, e0, prec, reorthog, generic):
+        p = dataclasses.replace(prob, aomega=_np(omega).copy(), concoef=_np(concoef).copy(),
+                                potterms=_np(potterms).copy(), shiftref=0)
+        eng = Engine(p)
+        if generic:
+            eng.select_hv_kernel(True)
+        eng.lanczos_init(niter, reorthog=reorthog, prec=prec, savevecs=True)
+        eng.run(1, niter)
+        a, b, _ = eng.coefficients()
+        x = _np(grid)
+        ctx.eng, ctx.args = eng, (a, b, x, sigma, origin, e0)
+        return torch.from_numpy(eng.spectrum(a, b, x, sigma, origin, e0))
+
+    @staticmethod
+    def backward(ctx, genv):
+        a, b, x, sigma, origin, e0 = ctx.args
+        _, _, tb = ctx.eng.spectrum_adjoint(a, b, x, sigma, _np(genv), origin, e0, full=True)
+        g = ctx.eng.krylov_gradient(tb)
+        ctx.eng.close()
+        return (torch.from_numpy(g["omega"]), torch.from_numpy(g["concoef"]), torch.from_numpy(g["potterms"]),
+                None, None, None, None, None, None, None, None, None)
+
+
+def spectrum_of_parameters(prob: Problem, omega, concoef, potterms, niter, grid, sigma, origin="lowest", e0=0.0,
+                           prec="f64", reorthog=0, generic=True):
+    """Broadened stick spectrum (convolve.py:28-48 without the division by the
+    maximum) of the niter-step run from the default start vector, on grid
+    (cm^-1), differentiable in omega, concoef and potterms through the
+    Krylov-projected gradient.  reorthog > 0 runs with partial
+    re-orthogonalisation; generic = True skips the run-time compile of the
+    specialised H*v (each call builds a new operator)."""
+    return SpectrumOfParameters.apply(omega, concoef, potterms, prob, niter, grid, sigma, origin, e0, prec, reorthog,
+                                      generic)

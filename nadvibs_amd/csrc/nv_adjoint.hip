@@ -199,7 +199,53 @@ __global__ void grad_finish_kernel(int nchunks, int nss, const double *__restric
     }
 }
 
+// H^T - H: only the inter-state vibrationally diagonal part can be
+// unsymmetric (the lower-only quirk masks D_ss'(n) above the state diagonal).
+// hub[s] += sum_{s' != s} (D_s's(n) - D_ss'(n)) ub[s'], D = dconst + sum_t h_t(n) dcmat_t.
+template <int NS>
+__global__ __launch_bounds__(256) void adj_transpose_kernel(NvOpDev op, const double *__restrict__ ub,
+                                                           double *__restrict__ hub) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= op.D) return;
+    double Dm[NS * NS];
+#pragma unroll
+    for (int e = 0; e < NS * NS; ++e) Dm[e] = op.dconst[e];
+    for (int t = 0; t < op.ndterms; ++t) {
+        const NvDiagTerm dt = op.dterms[t];
+        double h = 1.0;
+        for (int j = 0; j < dt.nf; ++j) {
+            int sh = 1;
+            for (int k = op.M - 1; k > dt.mode[j]; --k) sh *= op.nfunc[k];
+            h *= op.ftab64[dt.code[j] * op.nmax + (r / sh) % op.nfunc[dt.mode[j]]];
+        }
+#pragma unroll
+        for (int e = 0; e < NS * NS; ++e) Dm[e] = fma(h, op.dcmat[(size_t)t * NS * NS + e], Dm[e]);
+    }
+    double u[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) u[s] = ub[(int64_t)s * op.D + r];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double a = 0.0;
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2)
+            if (s2 != s) a = fma(Dm[s2 * NS + s] - Dm[s * NS + s2], u[s2], a);
+        hub[(int64_t)s * op.D + r] += a;
+    }
+}
+
 // ---- launchers ----------------------------------------------------------------
+hipError_t launch_adj_transpose(const NvOpDev &op, const double *ub, double *hub, hipStream_t s) {
+    const unsigned g = (unsigned)((op.D + 255) / 256);
+    switch (op.ns) {
+#define NV_T(n) case n: adj_transpose_kernel<n><<<g, 256, 0, s>>>(op, ub, hub); break;
+        NV_T(1) NV_T(2) NV_T(3) NV_T(4) NV_T(5) NV_T(6) NV_T(7) NV_T(8) NV_T(9) NV_T(10)
+#undef NV_T
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 int adj_blocks(int64_t N) {
     const int64_t nb = (N + 255) / 256;
     return (int)(nb < 2048 ? nb : 2048);
@@ -361,6 +407,39 @@ __global__ __launch_bounds__(256) void spec_gemm_kernel(int64_t n, const double 
         }
 }
 
+// Tb = P S^T (the full dL/dT), 32x32 tiles.
+__global__ __launch_bounds__(256) void spec_gemm_nt_kernel(int64_t n, const double *__restrict__ P,
+                                                          const double *__restrict__ S, double *__restrict__ Tb) {
+    __shared__ double As[32][33];
+    __shared__ double Bs[32][33];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int64_t row0 = (int64_t)blockIdx.y * 32, col0 = (int64_t)blockIdx.x * 32;
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int64_t k0 = 0; k0 < n; k0 += 32) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < 1024; t += 256) {
+            const int r = t / 32, cc = t % 32;
+            As[r][cc] = (row0 + r < n && k0 + cc < n) ? P[(row0 + r) * n + k0 + cc] : 0.0;
+            Bs[r][cc] = (col0 + r < n && k0 + cc < n) ? S[(col0 + r) * n + k0 + cc] : 0.0;  // Bs[col][k]
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < 32; ++kk) {
+            const double a0 = As[ty * 2][kk], a1 = As[ty * 2 + 1][kk];
+            const double b0 = Bs[tx * 2][kk], b1 = Bs[tx * 2 + 1][kk];
+            acc[0][0] = fma(a0, b0, acc[0][0]);
+            acc[0][1] = fma(a0, b1, acc[0][1]);
+            acc[1][0] = fma(a1, b0, acc[1][0]);
+            acc[1][1] = fma(a1, b1, acc[1][1]);
+        }
+    }
+    for (int u = 0; u < 2; ++u)
+        for (int v = 0; v < 2; ++v) {
+            const int64_t r = row0 + ty * 2 + u, cc = col0 + tx * 2 + v;
+            if (r < n && cc < n) Tb[r * n + cc] = acc[u][v];
+        }
+}
+
 // abar[j] = sum_k P[j][k] S[j][k]; bbar[j+1] = 2 sum_k P[j][k] S[j+1][k]: one block per row j.
 __global__ __launch_bounds__(256) void spec_rows_kernel(int64_t n, const double *__restrict__ S,
                                                        const double *__restrict__ P, double *__restrict__ abar,
@@ -388,7 +467,7 @@ hipError_t launch_spec_env(int64_t n, const double *theta, const double *vec, do
 
 hipError_t launch_spec_adjoint(int64_t n, const double *theta, const double *vec, double ref, int lowest, double c,
                                double sigma, int64_t ng, const double *grid, const double *gbar, double *work,
-                               double *abar, double *bbar, hipStream_t s) {
+                               double *abar, double *bbar, double *tbar, hipStream_t s) {
     double *W = work, *Wp = work + n, *ref0 = work + 2 * n, *M = work + 2 * n + 8, *P = M + n * n;
     spec_w_kernel<<<(unsigned)n, 256, 0, s>>>(theta, ref, c, sigma, ng, grid, gbar, W, Wp);
     if (lowest) spec_ref_kernel<<<1, 256, 0, s>>>(n, vec, Wp, ref0);
@@ -405,6 +484,7 @@ hipError_t launch_spec_adjoint(int64_t n, const double *theta, const double *vec
     const unsigned nt = (unsigned)((n + 31) / 32);
     spec_gemm_kernel<<<dim3(nt, nt), 256, 0, s>>>(n, vec, M, P);
     spec_rows_kernel<<<(unsigned)n, 256, 0, s>>>(n, vec, P, abar, bbar);
+    if (tbar) spec_gemm_nt_kernel<<<dim3(nt, nt), 256, 0, s>>>(n, P, vec, tbar);
     return hipGetLastError();
 }
 

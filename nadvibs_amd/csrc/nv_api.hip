@@ -604,8 +604,10 @@ int32_t nv_lanczos_run(nv_ctx *c, int64_t iiter, int64_t niter) {
     NV_HIP(c, hipSetDevice(c->device));
     const bool orthog = c->opts.reorthog > 0;
     for (int64_t i = iiter; i <= niter; ++i) {
-        // the first iteration and profiled runs go through the per-call path
-        if (i == 1 || c->prof) {
+        // the first iteration of a call and profiled runs go through the
+        // per-call path, which sets the device iteration counter that the
+        // captured graphs then advance (a resumed run starts past 1)
+        if (i == iiter || c->prof) {
             int32_t rc = nv_update_lanczos(c, i);
             if (!rc) rc = nv_hv_iter(c, i);
             if (!rc) rc = nv_lanczos_step(c, i);
@@ -1065,6 +1067,7 @@ int32_t nv_lanczos_adjoint(nv_ctx *c, int64_t n, const double *abar, const doubl
             e = nv::launch_adj_step(N, i, qb[ia], have_hub ? hub : nullptr, qn, sn, qi, qp, c->st.alpha, c->st.beta,
                                     abar_d, bbar_d, part, scal, ub, qb[ib], qb[ic], s);
             if (e == hipSuccess) e = hv_apply(c, prec, ub, hub, s);
+            if (e == hipSuccess && c->dev.offstate) e = nv::launch_adj_transpose(c->dev, ub, hub, s);
             if (e == hipSuccess) e = nv::launch_grad_contract(c->gdev, ub, qi, gpart, gacc, s);
             if (e == hipSuccess) e = nv::launch_adj_zero(N, qb[ia], s);
             have_hub = true;
@@ -1082,7 +1085,7 @@ int32_t nv_lanczos_adjoint(nv_ctx *c, int64_t n, const double *abar, const doubl
 
 static int32_t spectrum_common(nv_ctx *c, int64_t n, const double *alpha, const double *beta, int64_t ng,
                                const double *grid, double sigma, int32_t origin, double e0, const double *gbar,
-                               double *env, double *abar, double *bbar) {
+                               double *env, double *abar, double *bbar, double *tbar) {
     if (!c) return NV_ERR_ARG;
     if (n < 1 || ng < 1 || !alpha || !beta || !grid || !(sigma > 0) || (origin != 0 && origin != 1))
         return fail(c, NV_ERR_ARG, "bad spectrum arguments");
@@ -1099,6 +1102,7 @@ static int32_t spectrum_common(nv_ctx *c, int64_t n, const double *alpha, const 
     const double ref = origin == 0 ? root[0] : e0;
     const double cw = 219474.6313705;  // AU2WAVE (Source/progdata.f90:7)
     double *th = nullptr, *gr = nullptr, *gb = nullptr, *out = nullptr, *work = nullptr, *ab = nullptr, *bb = nullptr;
+    double *tb = nullptr;
     hipStream_t s = c->stream;
     hipError_t e = alloc(n * 8, (void **)&th);
     if (e == hipSuccess) e = alloc(ng * 8, (void **)&gr);
@@ -1113,12 +1117,14 @@ static int32_t spectrum_common(nv_ctx *c, int64_t n, const double *alpha, const 
         if (e == hipSuccess) e = alloc(((size_t)2 * n + 8 + (size_t)2 * n * n) * 8, (void **)&work);
         if (e == hipSuccess) e = alloc(n * 8, (void **)&ab);
         if (e == hipSuccess) e = alloc((n + 1) * 8, (void **)&bb);
+        if (tbar && e == hipSuccess) e = alloc((size_t)n * n * 8, (void **)&tb);
         if (e == hipSuccess) e = hipMemsetAsync(bb, 0, (n + 1) * 8, s);
         if (e == hipSuccess) e = hipMemcpyAsync(gb, gbar, ng * 8, hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
-            e = nv::launch_spec_adjoint(n, th, vec, ref, origin == 0, cw, sigma, ng, gr, gb, work, ab, bb, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(abar, ab, n * 8, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(bbar, bb, (n + 1) * 8, hipMemcpyDeviceToHost, s);
+            e = nv::launch_spec_adjoint(n, th, vec, ref, origin == 0, cw, sigma, ng, gr, gb, work, ab, bb, tb, s);
+        if (abar && e == hipSuccess) e = hipMemcpyAsync(abar, ab, n * 8, hipMemcpyDeviceToHost, s);
+        if (bbar && e == hipSuccess) e = hipMemcpyAsync(bbar, bb, (n + 1) * 8, hipMemcpyDeviceToHost, s);
+        if (tbar && e == hipSuccess) e = hipMemcpyAsync(tbar, tb, (size_t)n * n * 8, hipMemcpyDeviceToHost, s);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     for (void *p : tmp) hipFree(p);
@@ -1129,14 +1135,63 @@ static int32_t spectrum_common(nv_ctx *c, int64_t n, const double *alpha, const 
 int32_t nv_spectrum(nv_ctx *c, int64_t n, const double *alpha, const double *beta, int64_t ng, const double *grid,
                     double sigma, int32_t origin, double e0, double *env) {
     if (!env) return fail(c, NV_ERR_ARG, "env is NULL");
-    return spectrum_common(c, n, alpha, beta, ng, grid, sigma, origin, e0, nullptr, env, nullptr, nullptr);
+    return spectrum_common(c, n, alpha, beta, ng, grid, sigma, origin, e0, nullptr, env, nullptr, nullptr, nullptr);
 }
 
 int32_t nv_spectrum_adjoint(nv_ctx *c, int64_t n, const double *alpha, const double *beta, int64_t ng,
                             const double *grid, double sigma, int32_t origin, double e0, const double *gbar,
-                            double *abar, double *bbar) {
-    if (!gbar || !abar || !bbar) return fail(c, NV_ERR_ARG, "gbar/abar/bbar missing");
-    return spectrum_common(c, n, alpha, beta, ng, grid, sigma, origin, e0, gbar, nullptr, abar, bbar);
+                            double *abar, double *bbar, double *tbar) {
+    if (!gbar) return fail(c, NV_ERR_ARG, "gbar missing");
+    return spectrum_common(c, n, alpha, beta, ng, grid, sigma, origin, e0, gbar, nullptr, abar, bbar, tbar);
+}
+
+int32_t nv_krylov_gradient(nv_ctx *c, int64_t n, const double *tbar, double *g_omega, double *g_concoef,
+                           double *g_potterms) {
+    if (!c) return NV_ERR_ARG;
+    if (!tbar) return fail(c, NV_ERR_ARG, "tbar missing");
+    if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
+    if (c->opts.reorthog <= 0 && c->opts.savevecs == 0)
+        return fail(c, NV_ERR_STATE, "the Krylov gradient needs the kept basis (savevecs or reorthog)");
+    if (n < 1 || n > c->last_iter) return fail(c, NV_ERR_ARG, "n must lie in 1..last iteration");
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    int32_t rc = ensure_gop(c);
+    if (rc != NV_OK) return rc;
+    const int64_t N = c->op.N;
+    const int nss = (int)(c->op.ns * c->op.ns);
+    const int groups = nv::grad_groups(c->gdev), chunks = nv::grad_chunks(c->gdev);
+    const int64_t jc = std::min<int64_t>(n, 4 * nv::kRitzChunk);
+    std::vector<void *> tmp;
+    auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+        hipError_t er = hipMalloc(p, bytes);
+        if (er == hipSuccess) tmp.push_back(*p);
+        return er;
+    };
+    double *tb = nullptr, *W = nullptr, *gpart = nullptr, *gacc = nullptr;
+    hipError_t e = alloc((size_t)n * n * 8, (void **)&tb);
+    if (e == hipSuccess) e = alloc((size_t)jc * N * 8, (void **)&W);
+    if (e == hipSuccess) e = alloc((size_t)groups * chunks * nss * 8, (void **)&gpart);
+    if (e == hipSuccess) e = alloc((size_t)groups * nss * 8, (void **)&gacc);
+    hipStream_t s = c->stream;
+    if (e == hipSuccess) e = hipMemcpyAsync(tb, tbar, (size_t)n * n * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(gacc, 0, (size_t)groups * nss * 8, s);
+    // q_1 .. q_n are slots 1..n of the archive
+    const double *Q = c->st.V + N;
+    for (int64_t j0 = 0; j0 < n && e == hipSuccess; j0 += jc) {
+        const int64_t m = std::min(jc, n - j0);
+        // w_j = sum_k Tbar[j][k] q_k for j in [j0, j0+m): Tbar is symmetric, so its rows serve as
+        // the component-major coefficient table of launch_ritz_vectors
+        e = nv::launch_ritz_vectors(Q, N, n, tb + j0, m, W, s);
+        for (int64_t j = 0; j < m && e == hipSuccess; ++j)
+            e = nv::launch_grad_contract(c->gdev, W + j * N, Q + (j0 + j) * N, gpart, gacc, s);
+    }
+    std::vector<double> gh((size_t)groups * nss);
+    if (e == hipSuccess) e = hipMemcpyAsync(gh.data(), gacc, gh.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void *p : tmp) hipFree(p);
+    if (e != hipSuccess) return hip_fail(c, e, "krylov gradient");
+    map_gradients(c, gh, g_omega, g_concoef, g_potterms);
+    return NV_OK;
 }
 
 // ---- Gaussian broadening ----------------------------------------------------

@@ -98,6 +98,8 @@ hipError_t launch_adj_step(int64_t N, int64_t i, double *qbn, const double *hub,
                            const double *abar, double *bbar, double *part, double *scal, double *ub, double *qbi,
                            double *qbp, hipStream_t s);
 hipError_t launch_adj_zero(int64_t N, double *v, hipStream_t s);
+// hub += (H^T - H) ub: the unsymmetric inter-state diagonal part (lower-only quirk).
+hipError_t launch_adj_transpose(const NvOpDev &op, const double *ub, double *hub, hipStream_t s);
 // Bilinear forms ubar.(P_t q) of every monomial of g, added to gacc
 // [grad_groups][ns*ns]; part holds grad_groups * grad_chunks * ns*ns doubles.
 int grad_groups(const NvOpDev &g);
@@ -111,6 +113,6 @@ hipError_t launch_spec_env(int64_t n, const double *theta, const double *vec, do
                            int64_t ng, const double *grid, double *env, hipStream_t s);
 hipError_t launch_spec_adjoint(int64_t n, const double *theta, const double *vec, double ref, int lowest, double c,
                                double sigma, int64_t ng, const double *grid, const double *gbar, double *work,
-                               double *abar, double *bbar, hipStream_t s);
+                               double *abar, double *bbar, double *tbar, hipStream_t s);
 
 }  // namespace nv

@@ -224,13 +224,28 @@ class Engine:
                                          float(sigma), 0 if origin == "lowest" else 1, float(e0), dptr(env)))
         return env
 
-    def spectrum_adjoint(self, alpha, beta, grid, sigma, gbar, origin="lowest", e0=0.0):
+    def spectrum_adjoint(self, alpha, beta, grid, sigma, gbar, origin="lowest", e0=0.0, full=False):
+        """(abar, bbar) of L with dL/denv = gbar; with full=True also dL/dT
+        as a symmetric n x n matrix."""
         alpha, beta, grid, gbar = (np.ascontiguousarray(v, dtype=np.float64) for v in (alpha, beta, grid, gbar))
-        ab, bb = np.empty(alpha.size), np.empty(alpha.size + 1)
-        self._check(self.lib.nv_spectrum_adjoint(self.h, alpha.size, dptr(alpha), dptr(beta), grid.size, dptr(grid),
+        n = alpha.size
+        ab, bb = np.empty(n), np.empty(n + 1)
+        tb = np.empty((n, n)) if full else None
+        self._check(self.lib.nv_spectrum_adjoint(self.h, n, dptr(alpha), dptr(beta), grid.size, dptr(grid),
                                                  float(sigma), 0 if origin == "lowest" else 1, float(e0), dptr(gbar),
-                                                 dptr(ab), dptr(bb)))
-        return ab, bb
+                                                 dptr(ab), dptr(bb), dptr(tb)))
+        return (ab, bb, tb) if full else (ab, bb)
+
+    def krylov_gradient(self, tbar):
+        """dL/d(aomega, concoef, potterms) of a spectral loss with dL/dT = tbar,
+        projected on the kept Krylov basis (nv_krylov_gradient)."""
+        tbar = np.ascontiguousarray(tbar, dtype=np.float64)
+        n = tbar.shape[0]
+        p = self.prob
+        go, gc = np.zeros(p.nmodes), np.zeros(p.nstblks)
+        gp = np.zeros(np.shape(p.potterms))
+        self._check(self.lib.nv_krylov_gradient(self.h, n, dptr(tbar), dptr(go), dptr(gc), dptr(gp)))
+        return dict(omega=go, concoef=gc, potterms=gp)
 
     def hv_kernel(self, prec="f64"):
         """Which H*v kernel serves `prec` (compiles it on first use)."""

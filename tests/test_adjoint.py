@@ -9,8 +9,9 @@ import numpy as np
 import pytest
 
 from nadvibs_amd.synth import make_problem
-from oracle.adjoint import (complex_step_lanczos, lanczos_adjoint, lanczos_basis, parameter_matrices, spectrum,
-                            spectrum_adjoint)
+from oracle.adjoint import (complex_step_lanczos, krylov_gradient, lanczos_adjoint, lanczos_basis, parameter_matrices,
+                            spectrum, spectrum_adjoint)
+from oracle.oracle import Oracle
 from oracle.dense import dense_hamiltonian
 
 
@@ -95,3 +96,46 @@ def test_spectrum_adjoint_finite_on_ghost_pair():
     h = 1e-10
     fa = np.array([(L(a + h * e, b) - L(a - h * e, b)) / (2 * h) for e in np.eye(a.size)])
     assert np.max(np.abs(ab - fa)) <= 1e-4 * np.max(np.abs(fa))
+
+
+@pytest.mark.parametrize("reorthog", [0, 1])
+def test_krylov_gradient_matches_central_differences_of_the_lanczos_loss(reorthog):
+    """The projected gradient of a broadened-spectrum loss against central
+    differences of the oracle's own Lanczos run (exact operator; with and
+    without re-orthogonalisation, ghosts included), at 80 iterations."""
+    import copy
+    p = _prob(ns=3, nf=(6, 5, 4, 4), seed=34, quirks=0)
+    H = dense_hamiltonian(p)
+    mats = parameter_matrices(p)
+    grid = np.linspace(-100.0, 4000.0, 400)
+    n = 80
+
+    def loss(pp):
+        r = Oracle(pp).lanczos(n, reorthog=reorthog, keep_basis=True)
+        env = spectrum(r["alpha"], r["beta"], grid, 30.0, "fixed", 0.0)
+        return float(env @ env), r, env
+
+    L, r, env = loss(p)
+    _, _, Tb = spectrum_adjoint(r["alpha"], r["beta"], grid, 30.0, 2 * env, "fixed", 0.0, full=True)
+    M, nb = p.nmodes, p.nstblks
+    pick = [0, 2, M, M + 1] + [M + nb + int(k) for k in np.flatnonzero(np.abs(p.potterms.reshape(-1)) > 0)[:6]]
+    g = krylov_gradient(r["basis"], Tb, [mats[i] for i in pick])
+    fd = []
+    for i in pick:
+        def moved(h):
+            pp = copy.deepcopy(p)
+            if i < M:
+                pp.aomega = np.array(p.aomega, dtype=float)
+                pp.aomega[i] += h
+            elif i < M + nb:
+                pp.concoef = np.array(p.concoef, dtype=float)
+                pp.concoef[i - M] += h
+            else:
+                pp.potterms = p.potterms.copy()
+                pp.potterms.reshape(-1)[i - M - nb] += h
+            return pp
+        v = (list(p.aomega) + list(p.concoef) + list(p.potterms.reshape(-1)))[i]
+        h = 1e-6 * max(abs(v), 1e-4)
+        fd.append((loss(moved(h))[0] - loss(moved(-h))[0]) / (2 * h))
+    fd = np.array(fd)
+    assert np.max(np.abs(g - fd)) <= 1e-6 * np.max(np.abs(fd))

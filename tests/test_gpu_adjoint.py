@@ -1,21 +1,25 @@
-"""GPU: the library's reverse sweep and spectrum adjoint against the oracle
-(oracle/adjoint.py, itself pinned by complex-step derivatives), checkpoint
-recomputation against the kept basis, mixed precision against FP64, and the
-torch binding end to end against central differences.
+"""GPU: the library's reverse sweep, Krylov-projected gradient and spectrum
+adjoint against the oracle (oracle/adjoint.py, itself pinned by complex-step
+derivatives and central differences), checkpoint recomputation against the
+kept basis, mixed precision against FP64, and the torch binding end to end
+against central differences of the device forward.
 
-Tolerances: FP64 adjoint vs oracle 1e-9 of the largest gradient component
-(the forward runs differ by rounding only); checkpointed vs kept basis
-exactly equal (the recomputation is checked bit for bit); FP32 H*v vs FP64
-1e-5 of the largest component; central differences 1e-5.
+Tolerances: FP64 reverse sweep vs oracle 1e-9 of the largest gradient
+component (short runs: the forward runs differ by rounding only);
+checkpointed vs kept basis exactly equal (the recomputation is checked bit
+for bit); Krylov gradient vs oracle on the same basis 1e-10; FP32 H*v vs FP64
+1e-5 of the largest component; central differences 1e-5 (short runs) and
+1e-6 (the Krylov gradient at 80 iterations).
 """
 import numpy as np
 import pytest
 import torch
 
-from nadvibs_amd.adjoint import broadened_spectrum, lanczos_coefficients
+from nadvibs_amd.adjoint import broadened_spectrum, lanczos_coefficients, spectrum_of_parameters
 from nadvibs_amd.engine import Engine
 from nadvibs_amd.synth import make_problem
-from oracle.adjoint import lanczos_adjoint, lanczos_basis, parameter_matrices, spectrum, spectrum_adjoint
+from oracle.adjoint import (krylov_gradient, lanczos_adjoint, lanczos_basis, parameter_matrices, spectrum,
+                            spectrum_adjoint)
 from oracle.dense import dense_hamiltonian
 
 pytestmark = pytest.mark.gpu
@@ -79,17 +83,6 @@ def test_adjoint_checkpoints_equal_kept_basis():
             np.testing.assert_array_equal(g_cp[key], g_keep[key])
 
 
-def test_adjoint_mixed_precision_within_1e5_of_fp64():
-    p = _prob(ns=3, nf=(8, 7, 6, 5), seed=34)
-    n = 30
-    rng = np.random.default_rng(9)
-    abar, bbar = rng.standard_normal(n), rng.standard_normal(n + 1)
-    _, _, _, g64 = _gpu_grad(p, n, abar, bbar, "f64")
-    _, _, _, g32 = _gpu_grad(p, n, abar, bbar, "f32")
-    f64, f32 = _flat(g64, p), _flat(g32, p)
-    assert np.max(np.abs(f32 - f64)) <= 1e-5 * np.max(np.abs(f64))
-
-
 @pytest.mark.parametrize("origin", ["lowest", "fixed"])
 def test_spectrum_and_adjoint_match_oracle(origin):
     p = _prob()
@@ -150,3 +143,74 @@ def test_torch_loss_gradient_matches_central_differences():
         got = float(vals[name].grad[idx])
         assert abs(got - fd) <= 1e-5 * max(abs(fd), 1e-3 * abs(float(L))), (name, idx, got, fd)
     dev.close()
+
+
+def _spec_loss_grad(p, n, prec, reorthog=0):
+    grid = np.linspace(-100.0, 4000.0, 400)
+    with Engine(p) as eng:
+        eng.lanczos_init(n, reorthog=reorthog, prec=prec, savevecs=True)
+        eng.run(1, n)
+        a, b, _ = eng.coefficients()
+        env = eng.spectrum(a, b, grid, 30.0, "fixed", 0.0)
+        _, _, tb = eng.spectrum_adjoint(a, b, grid, 30.0, 2 * env, "fixed", 0.0, full=True)
+        g = eng.krylov_gradient(tb)
+        basis = np.array([eng.lanczos_vector(j) for j in range(1, n + 1)])
+    return g, tb, basis
+
+
+@pytest.mark.parametrize("reorthog", [0, 1])
+def test_krylov_gradient_matches_oracle_on_the_same_basis(reorthog):
+    p = _prob(ns=3, nf=(4, 4, 3, 3), seed=36, quirks=0)
+    n = 60
+    g, tb, basis = _spec_loss_grad(p, n, "f64", reorthog)
+    mats = [m for m in parameter_matrices(p) if m is not None]
+    ref = krylov_gradient(basis, tb, mats)
+    got = _flat(g, p)
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+def test_krylov_gradient_mixed_precision_within_1e5_of_fp64():
+    p = _prob(ns=3, nf=(8, 7, 6, 5), seed=34, quirks=0)
+    g64, _, _ = _spec_loss_grad(p, 120, "f64")
+    g32, _, _ = _spec_loss_grad(p, 120, "f32")
+    f64, f32 = _flat(g64, p), _flat(g32, p)
+    assert np.max(np.abs(f32 - f64)) <= 1e-5 * np.max(np.abs(f64))
+
+
+def test_spectrum_of_parameters_matches_central_differences():
+    """The stable torch path at 80 iterations against central differences of
+    the device forward (exact operator)."""
+    p = _prob(ns=3, nf=(6, 5, 4, 4), seed=37, quirks=0)
+    n = 80
+    grid = torch.linspace(-100.0, 4000.0, 400, dtype=torch.float64)
+    target = torch.zeros(grid.numel(), dtype=torch.float64)
+    omega = torch.tensor(p.aomega, dtype=torch.float64, requires_grad=True)
+    concoef = torch.tensor(p.concoef, dtype=torch.float64, requires_grad=True)
+    potterms = torch.tensor(p.potterms, dtype=torch.float64, requires_grad=True)
+
+    def loss_of(w, c, pt):
+        env = spectrum_of_parameters(p, w, c, pt, n, grid, 30.0, "fixed", 0.0)
+        return torch.sum((env - target) ** 2)
+
+    L = loss_of(omega, concoef, potterms)
+    L.backward()
+    rng = np.random.default_rng(12)
+    nz = np.argwhere(np.abs(p.potterms) > 0)
+    checks = [("omega", (k,)) for k in range(p.nmodes)] + [("concoef", (0,)), ("concoef", (2,))]
+    checks += [("potterms", tuple(nz[i])) for i in rng.choice(len(nz), 6, replace=False)]
+    vals = {"omega": omega, "concoef": concoef, "potterms": potterms}
+    names = ("omega", "concoef", "potterms")
+    fds, gots = [], []
+    for name, idx in checks:
+        x = float(vals[name].detach()[idx])
+        h = 1e-6 * max(abs(x), 1e-4)
+        up = [vals[k].detach().clone() for k in names]
+        dn = [vals[k].detach().clone() for k in names]
+        j = names.index(name)
+        up[j][idx] += h
+        dn[j][idx] -= h
+        with torch.no_grad():
+            fds.append((float(loss_of(*up)) - float(loss_of(*dn))) / (2 * h))
+        gots.append(float(vals[name].grad[idx]))
+    fds, gots = np.array(fds), np.array(gots)
+    assert np.max(np.abs(gots - fds)) <= 1e-6 * np.max(np.abs(fds)), (gots, fds)
