@@ -122,9 +122,10 @@ typedef struct nv_lanczos_opts {
                                  omega recurrence, >1 partial with exact overlaps */
     int32_t prec;             /* NV_PREC_* for H*v */
     int64_t chkorthog;        /* exact overlap check period (default 100); with
-                                 NV_PREC_F32/BF16 the period is min(chkorthog, 10):
-                                 the omega model assumes FP64 rounding of H*v and
-                                 the checks keep it honest */
+                                 NV_PREC_F32/BF16 and pro_eps = 0 the period is
+                                 min(chkorthog, 10): the omega model then assumes
+                                 FP64 rounding of H*v and the checks keep it
+                                 honest */
     uint64_t seed;            /* seed of the omega-estimate noise */
     double noise_scale;       /* multiplies every noise draw (1 = reference, 0 = off) */
     double pro_eps;           /* unit roundoff assumed by the omega recurrence and

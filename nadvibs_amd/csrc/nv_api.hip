@@ -288,10 +288,12 @@ static hipError_t hv_apply(nv_ctx *c, int32_t prec, const double *x, double *y, 
     const void *xin = x;
     if (prec == NV_PREC_F32) {
         if (!c->hx32) {
-            hipError_t e = hipMalloc(&c->hx32, (size_t)c->op.N * sizeof(float));
+            const size_t bytes = (size_t)c->op.D * nv_lo_stride((int)c->op.ns) * sizeof(float);
+            hipError_t e = hipMalloc(&c->hx32, bytes);
+            if (e == hipSuccess) e = hipMemset(c->hx32, 0, bytes);
             if (e != hipSuccess) return e;
         }
-        hipError_t e = nv::launch_cast_f32(x, c->hx32, c->op.N, s);
+        hipError_t e = nv::launch_cast_f32(x, c->hx32, c->op.N, c->op.D, (int)c->op.ns, s);
         if (e != hipSuccess) return e;
         xin = c->hx32;
     }
@@ -411,7 +413,11 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
         };
         NV_HIP(c, alloc((size_t)st.nslots * N * sizeof(double), (void **)&st.V));
         const size_t el = o->prec == NV_PREC_F64 ? 8 : (o->prec == NV_PREC_F32 ? 4 : 2);
-        if (o->prec != NV_PREC_F64) NV_HIP(c, alloc((size_t)N * el, &st.vlo));
+        if (o->prec != NV_PREC_F64) {
+            const size_t lo_bytes = (size_t)c->op.D * nv_lo_stride((int)c->op.ns) * el;
+            NV_HIP(c, alloc(lo_bytes, &st.vlo));
+            NV_HIP(c, hipMemset(st.vlo, 0, lo_bytes));
+        }
         NV_HIP(c, alloc((size_t)N * (o->prec == NV_PREC_F64 ? 8 : 4), &st.y));
         NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.alpha));
         NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.beta));

@@ -98,12 +98,21 @@ __device__ __forceinline__ double nv_block_sum(double v) {
     return red[0] + red[1] + red[2] + red[3];
 }
 )";
-    if (f32)
-        s << "__device__ __forceinline__ T nv_ld(__amdgpu_buffer_rsrc_t r, unsigned o) { return "
-             "__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)o, 0, 0)); }\n";
-    else
+    const int nsp = nv_lo_stride(ns);
+    if (f32) {
+        // the FP32 copy is state-interleaved, [D][nsp]: one vector load per step
+        s << "typedef float nv_f2 __attribute__((ext_vector_type(2)));\n"
+             "typedef float nv_f4 __attribute__((ext_vector_type(4)));\n"
+             "__device__ __forceinline__ float nv_ld1(__amdgpu_buffer_rsrc_t r, unsigned o) { return "
+             "__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)o, 0, 0)); }\n"
+             "__device__ __forceinline__ nv_f2 nv_ld2(__amdgpu_buffer_rsrc_t r, unsigned o) { return "
+             "__builtin_bit_cast(nv_f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o, 0, 0)); }\n"
+             "__device__ __forceinline__ nv_f4 nv_ld4(__amdgpu_buffer_rsrc_t r, unsigned o) { return "
+             "__builtin_bit_cast(nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)o, 0, 0)); }\n";
+    } else {
         s << "__device__ __forceinline__ T nv_ld(__amdgpu_buffer_rsrc_t r, unsigned o) { return "
              "__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o, 0, 0)); }\n";
+    }
     // the body shared by both kernels: needs `x` (const T*), `r`, writes acc_s
     std::ostringstream b;
     b << "    __shared__ T tab[" << (tab.empty() ? 1 : tab.size()) << "];\n";
@@ -117,12 +126,40 @@ __device__ __forceinline__ double nv_block_sum(double v) {
     }
     for (auto &kv : codes)
         b << "    const T f" << kv.first << " = tab[" << kv.second.second << " + d" << kv.second.first << "];\n";
-    for (int q = 0; q < ns; ++q)
-        b << "    const __amdgpu_buffer_rsrc_t rs" << q << " = __builtin_amdgcn_make_buffer_rsrc((void *)(x + "
-          << q << "LL * NV_D), (short)0, (int)(NV_D * sizeof(T)), 0x00020000);\n";
-    b << "    const unsigned vo = (unsigned)r * (unsigned)sizeof(T);\n";
+    if (f32) {
+        b << "    const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)(NV_D * "
+          << nsp * 4 << "), 0x00020000);\n";
+        b << "    const unsigned vo = (unsigned)r * " << nsp * 4 << "u;\n";
+    } else {
+        for (int q = 0; q < ns; ++q)
+            b << "    const __amdgpu_buffer_rsrc_t rs" << q << " = __builtin_amdgcn_make_buffer_rsrc((void *)(x + "
+              << q << "LL * NV_D), (short)0, (int)(NV_D * sizeof(T)), 0x00020000);\n";
+        b << "    const unsigned vo = (unsigned)r * (unsigned)sizeof(T);\n";
+    }
     for (int q = 0; q < ns; ++q) b << "    T acc" << q << " = 0;\n";
-    const size_t el = f32 ? 4 : 8;
+    const size_t el = f32 ? (size_t)nsp * 4 : 8;
+    // the loads of one step: w_q += h * x[q][r + stride]
+    auto step_loads = [&](std::ostringstream &o) {
+        if (!f32) {
+            for (int q = 0; q < ns; ++q) o << " w" << q << " = __builtin_fma(h, nv_ld(rs" << q << ", o), w" << q << ");";
+            return;
+        }
+        if (nsp == 1) {
+            o << " w0 = __builtin_fma(h, nv_ld1(rx, o), w0);";
+        } else if (nsp == 2) {
+            o << " { const nv_f2 v = nv_ld2(rx, o);";
+            for (int q = 0; q < ns; ++q) o << " w" << q << " = __builtin_fma(h, v[" << q << "], w" << q << ");";
+            o << " }";
+        } else {
+            for (int part = 0; part < nsp / 4; ++part) {
+                if (part * 4 >= ns) break;
+                o << " { const nv_f4 v = nv_ld4(rx, o + " << part * 16 << "u);";
+                for (int q = part * 4; q < ns && q < part * 4 + 4; ++q)
+                    o << " w" << q << " = __builtin_fma(h, v[" << q - part * 4 << "], w" << q << ");";
+                o << " }";
+            }
+        }
+    };
     for (size_t ti = 0; ti < op.dev_terms.size(); ++ti) {
         const NvTerm &tm = op.dev_terms[ti];
         b << "    {\n";
@@ -132,7 +169,7 @@ __device__ __forceinline__ double nv_block_sum(double v) {
             b << "        { const unsigned o = vo + " << (uint32_t)((int64_t)st.stride * (int64_t)el) << "u; const T h = ";
             for (int j = 0; j < st.nf; ++j) b << (j ? " * " : "") << "f" << st.code[j];
             b << ";";
-            for (int q = 0; q < ns; ++q) b << " w" << q << " = __builtin_fma(h, nv_ld(rs" << q << ", o), w" << q << ");";
+            step_loads(b);
             b << " }\n";
         }
         const double *C = op.cmat.data() + ti * ns * ns;

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <type_traits>
+
 #include "nv_kernels.hpp"
 
 namespace nv {
@@ -188,8 +190,15 @@ __device__ __forceinline__ void offdiag_accumulate(const NvOpDev &op, const TIn 
                 if (j < st.nf) h *= ftab[st.code[j] * op.nmax + dig[st.mode[j] * kBlock + tid]];
             if (h != (TAcc)0) {
                 const int idx = r + st.stride;
+                if constexpr (std::is_same<TIn, double>::value) {
 #pragma unroll
-                for (int s = 0; s < NS; ++s) w[s] += h * load_acc<TIn, TAcc>(x + (size_t)s * op.D + idx);
+                    for (int s = 0; s < NS; ++s) w[s] += h * load_acc<TIn, TAcc>(x + (size_t)s * op.D + idx);
+                } else {
+                    // interleaved low-precision copy: the states of a row are adjacent
+                    const TIn *xr = x + (size_t)idx * nv_lo_stride(NS);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) w[s] += h * load_acc<TIn, TAcc>(xr + s);
+                }
             }
         }
         const TAcc *C = cm + (size_t)t * NS * NS;
@@ -310,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzStat
 // update_lanczos (Source/top.f90:646-670): q_i = r / beta(i-1) in place, and
 // the copy of q_i in the H*v input precision.
 template <typename TIn>
-__global__ __launch_bounds__(kBlock) void normalize_kernel(int64_t N, LzState st) {
+__global__ __launch_bounds__(kBlock) void normalize_kernel(int64_t N, int64_t D, int nsp, LzState st) {
     const int64_t it = *st.iter;
     double *q = st.V + slot_of(it, st.nslots) * N;
     const double sc = 1.0 / st.beta[it - 1];
@@ -318,7 +327,10 @@ __global__ __launch_bounds__(kBlock) void normalize_kernel(int64_t N, LzState st
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock) {
         const double v = q[i] * sc;
         q[i] = v;
-        if (lo) lo[i] = from_d<TIn>(v);
+        if (lo) {
+            const int64_t s = i / D;
+            lo[(i - s * D) * nsp + s] = from_d<TIn>(v);
+        }
     }
 }
 
@@ -462,22 +474,28 @@ hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, h
 
 hipError_t launch_normalize(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     const int g = grid_stride_blocks(op.N);
+    const int64_t N = op.N, D = op.D;
+    const int nsp = nv_lo_stride(op.ns);
     if (prec == 0 || st.vlo == nullptr)
-        hipLaunchKernelGGL((normalize_kernel<double>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N, st);
+        hipLaunchKernelGGL((normalize_kernel<double>), dim3(g), dim3(kBlock), 0, s, N, D, nsp, st);
     else if (prec == 1)
-        hipLaunchKernelGGL((normalize_kernel<float>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N, st);
+        hipLaunchKernelGGL((normalize_kernel<float>), dim3(g), dim3(kBlock), 0, s, N, D, nsp, st);
     else
-        hipLaunchKernelGGL((normalize_kernel<__hip_bfloat16>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N, st);
+        hipLaunchKernelGGL((normalize_kernel<__hip_bfloat16>), dim3(g), dim3(kBlock), 0, s, N, D, nsp, st);
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kBlock) void cast_f32_kernel(const double *x, float *y, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        y[i] = (float)x[i];
+// y[r * nsp + s] = (float)x[s * D + r] (the interleaved FP32 copy).
+__global__ __launch_bounds__(kBlock) void cast_f32_kernel(const double *x, float *y, int64_t n, int64_t D, int nsp) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const int64_t s = i / D;
+        y[(i - s * D) * nsp + s] = (float)x[i];
+    }
 }
 
-hipError_t launch_cast_f32(const double *x, float *y, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(cast_f32_kernel, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, x, y, n);
+hipError_t launch_cast_f32(const double *x, float *y, int64_t n, int64_t D, int ns, hipStream_t s) {
+    hipLaunchKernelGGL(cast_f32_kernel, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, x, y, n, D,
+                       nv_lo_stride(ns));
     return hipGetLastError();
 }
 

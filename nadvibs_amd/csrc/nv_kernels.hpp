@@ -15,7 +15,8 @@ constexpr int kBlock = 256;
 struct LzState {
     double *V = nullptr;
     int64_t nslots = 0;
-    void *vlo = nullptr;         // q_i in the H*v input precision (F32/BF16)
+    void *vlo = nullptr;         // q_i in the H*v input precision (F32/BF16), interleaved
+                                 // [D][nv_lo_stride(ns)], padding zero
     void *y = nullptr;           // off-diagonal part of H q_i (accumulation type)
     double *alpha = nullptr;     // [niter+2], alpha[i]
     double *beta = nullptr;      // [niter+2], beta[i], beta[0] = |start|
@@ -34,8 +35,8 @@ int hv_blocks(const NvOpDev &op);
 hipError_t launch_diag_precompute(const NvOpDev &op, double *dg, hipStream_t s);
 // y = H x, x and y double, full operator (standalone H*v).
 hipError_t launch_hv_full(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s);
-// y = (float)x.
-hipError_t launch_cast_f32(const double *x, float *y, int64_t n, hipStream_t s);
+// The interleaved FP32 copy: y[r * nv_lo_stride(ns) + s] = (float)x[s * D + r].
+hipError_t launch_cast_f32(const double *x, float *y, int64_t n, int64_t D, int ns, hipStream_t s);
 // Sets (or increments) the device iteration counter.
 hipError_t launch_set_iter(int64_t *d_iter, int64_t value, int increment, hipStream_t s);
 // update_lanczos: q_i = r_{i-1} / beta(i-1), plus the low-precision copy.

@@ -409,9 +409,11 @@ __global__ __launch_bounds__(kBlock) void pro_beta_kernel(ProDev p, LzState st, 
 // Exact-check period: the reference's chkorthog; in the mixed modes at most
 // 10, since the omega recurrence models FP64 rounding of H*v and the FP32
 // couplings lose orthogonality faster between checks.
+// Mixed modes with the default (FP64) roundoff model check the overlaps every
+// min(chkorthog, 10) iterations; a caller that sets pro_eps chooses the period.
 static int64_t chk_period(const nv_lanczos_opts &o) {
     int64_t c = o.chkorthog < 1 ? 100 : o.chkorthog;
-    if (o.prec != NV_PREC_F64 && c > 10) c = 10;
+    if (o.prec != NV_PREC_F64 && !(o.pro_eps > 0) && c > 10) c = 10;
     return c;
 }
 

@@ -122,13 +122,21 @@ class Oracle:
         return out
 
     def lanczos(self, niter, reorthog=0, chkorthog=100, seed=12345, noise=1.0, start=None, keep_basis=False,
-                mixed=False):
+                mixed=False, pro_eps=0.0):
         """mixed=True emulates the library's mixed mode: off-diagonal part of
-        H*v from a float copy with float accumulation, exact overlap checks
-        every min(chkorthog, 10) iterations."""
+        H*v from a float copy with float accumulation; with the default
+        pro_eps (0: 2^-53) exact overlap checks every min(chkorthog, 10)
+        iterations.  pro_eps > 0 sets the roundoff of the omega model."""
+        if pro_eps:
+            set_eps_pro(pro_eps)
+            try:
+                return self.lanczos(niter, reorthog, chkorthog, seed, noise, start, keep_basis, mixed)
+            finally:
+                set_eps_pro(0.0)
         if mixed:
             lib().or_set_off32(1)
-            chkorthog = min(chkorthog, 10)
+            if not lib_eps_set():
+                chkorthog = min(chkorthog, 10)
             try:
                 return self.lanczos(niter, reorthog, chkorthog, seed, noise, start, keep_basis, mixed=False)
             finally:
@@ -155,9 +163,17 @@ def set_off32(on):
     lib().or_set_off32(1 if on else 0)
 
 
+_EPS_PRO = [0.0]
+
+
 def set_eps_pro(eps):
     """Unit roundoff of the omega recurrence (0 = the reference's 2^-53)."""
+    _EPS_PRO[0] = float(eps)
     lib().or_set_eps_pro(eps)
+
+
+def lib_eps_set():
+    return _EPS_PRO[0] > 0.0
 
 
 def givens(alpha, beta, vectors=False):
