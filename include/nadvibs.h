@@ -259,6 +259,32 @@ int32_t nv_krylov_gradient(nv_ctx *ctx, int64_t n, const double *tbar,
  * once, e.g. inside a fit).  Call before nv_lanczos_init. */
 int32_t nv_select_hv_kernel(nv_ctx *ctx, int32_t generic);
 
+/* ---- batched runs (no counterpart in the reference: many Hamiltonians,
+ *      e.g. parameter sets or basis truncations, advance together) --------- */
+/* One context per problem (operator tables, start vector, eigensolve via
+ * nv_batch_context + nv_ritz); the iteration itself runs as one launch
+ * sequence over every problem (tiles of rows, one workgroup each), captured
+ * once as a HIP graph.  reorthog = 0, the table-driven H*v kernels. */
+typedef struct nv_batch nv_batch;
+int32_t nv_batch_create(const nv_problem *probs, int64_t count, int32_t device,
+                        nv_batch **out);
+int32_t nv_batch_destroy(nv_batch *b);
+const char *nv_batch_last_error(const nv_batch *b);
+/* Allocates the vectors of niter iterations in precision prec (NV_PREC_F64
+ * or NV_PREC_F32) and loads every start vector. */
+int32_t nv_batch_init(nv_batch *b, int64_t niter, int32_t prec);
+/* Times the batched H*v at 64, 128 and 256 rows per workgroup and keeps the
+ * fastest; call between nv_batch_init and the run.  report (may be NULL)
+ * receives the timings. */
+int32_t nv_batch_tune(nv_batch *b, char *report, int64_t len);
+/* Iterations iiter..niter for every problem.  Asynchronous. */
+int32_t nv_batch_run(nv_batch *b, int64_t iiter, int64_t niter);
+/* alpha(1:n) and beta(0:n) of problem p (n = last iteration).  Synchronises. */
+int32_t nv_batch_coefficients(nv_batch *b, int64_t p, double *alpha,
+                              double *beta);
+/* The context of problem p (owned by the batch). */
+nv_ctx *nv_batch_context(nv_batch *b, int64_t p);
+
 /* ---- spectrum broadening (convolve.py:28-48, the Gaussian aperture) ----- */
 /* env[g] = sum_j y[j] exp(-((grid[g] - x[j]) / sigma)^2 / 2), divided by its
  * maximum over the grid (sigma is a standard deviation, not a FWHM). */
@@ -286,6 +312,10 @@ int32_t nv_profile_enable(nv_ctx *ctx, int32_t on);
 /* Milliseconds spent in the timed H*v launches and their number since the
  * previous call; synchronises. */
 int32_t nv_profile_read(nv_ctx *ctx, double *hv_ms, int64_t *hv_launches);
+/* Average duration in microseconds of `launches` back-to-back launches of
+ * the iteration H*v kernel alone (after nv_update_lanczos(1)), timed with one
+ * pair of HIP events on the context's stream.  Overwrites alpha(1). */
+int32_t nv_profile_hv(nv_ctx *ctx, int64_t launches, double *avg_us);
 /* Which H*v kernel serves precision prec: the operator-specialised one
  * (compiled at run time, on first use) or the generic one, as text.  The
  * environment variable NV_HV_KERNEL=generic selects the generic kernel. */

@@ -75,6 +75,16 @@ SIGNATURES = [
                                              ctypes.c_double, ctypes.c_int32, ctypes.c_double, _dp, _dp, _dp, _dp]),
     ("nv_krylov_gradient", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, _dp]),
     ("nv_select_hv_kernel", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    ("nv_profile_hv", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp]),
+    ("nv_batch_create", ctypes.c_int32, [ctypes.POINTER(NvProblem), ctypes.c_int64, ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    ("nv_batch_destroy", ctypes.c_int32, [ctypes.c_void_p]),
+    ("nv_batch_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
+    ("nv_batch_init", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32]),
+    ("nv_batch_tune", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
+    ("nv_batch_run", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64]),
+    ("nv_batch_coefficients", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp]),
+    ("nv_batch_context", ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int64]),
     ("nv_identify_roots", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, ctypes.c_int64,
                                            _dp, _dp, _i32p, _i64p, _i32p, _i32p, _dp, _dp, _dp, _dp]),
     ("nv_convolve", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp,

@@ -2,6 +2,7 @@
 // tables on the device, the Krylov basis and every scratch buffer.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -510,6 +511,32 @@ int32_t nv_hv_kernel_info(nv_ctx *c, int32_t prec, char *buf, int64_t len) {
 int32_t nv_profile_enable(nv_ctx *c, int32_t on) {
     if (!c) return NV_ERR_ARG;
     c->prof = on != 0;
+    return NV_OK;
+}
+
+int32_t nv_profile_hv(nv_ctx *c, int64_t launches, double *avg_us) {
+    if (!c || !avg_us || launches < 1) return NV_ERR_ARG;
+    if (!c->lz_ready || c->next_update < 2) return fail(c, NV_ERR_STATE, "update_lanczos(1) must come first");
+    NV_HIP(c, hipSetDevice(c->device));
+    const nv::JitKernels *k = ensure_jit(c, c->opts.prec);
+    auto one = [&]() {
+        return k ? nv::jit_launch_iter(*k, c->dev, c->st, c->stream)
+                 : nv::launch_hv_iter_only(c->dev, c->opts.prec, c->st, c->stream);
+    };
+    hipEvent_t a, b;
+    NV_HIP(c, hipEventCreate(&a));
+    NV_HIP(c, hipEventCreate(&b));
+    hipError_t e = one();  // warm
+    if (e == hipSuccess) e = hipEventRecord(a, c->stream);
+    for (int64_t i = 0; i < launches && e == hipSuccess; ++i) e = one();
+    if (e == hipSuccess) e = hipEventRecord(b, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (e != hipSuccess) return hip_fail(c, e, "profile_hv");
+    *avg_us = 1000.0 * ms / (double)launches;
     return NV_OK;
 }
 
@@ -1271,6 +1298,326 @@ int32_t nv_load_vectors(nv_ctx *c, const char *path, int64_t iter, const double 
     NV_HIP(c, hipMemcpy(c->st.beta, beta, (iter + 1) * sizeof(double), hipMemcpyHostToDevice));
     c->last_iter = iter;
     c->next_update = iter + 1;
+    return NV_OK;
+}
+
+}  // extern "C"
+
+// ---- batched runs ----------------------------------------------------------------
+// Many problems advance together: one launch sequence per iteration covers
+// every problem (tiles of bs rows, one workgroup each), captured once as a
+// graph and replayed.  Each problem keeps its own context for its operator
+// tables, start vector and eigensolve.
+struct nv_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<nv_ctx *> ctx;
+    std::vector<int64_t> voff, loff;
+    int64_t Ntot = 0, Ltot = 0;
+    int prec = NV_PREC_F64;
+    int64_t niter = 0, last = 0, next = 1;
+    bool ready = false;
+    nv::BatchDev d{};
+    NvOpDev *ops_d = nullptr;
+    int64_t *voff_d = nullptr, *loff_d = nullptr, *iter_d = nullptr;
+    nv::BatchTile *tiles_d = nullptr;
+    int32_t *tile0_d = nullptr;
+    std::vector<void *> allocs, run_allocs;
+    size_t lds_hv = 0, lds_step = 0;
+    hipGraphExec_t graph = nullptr;
+    int bs = 256;
+};
+
+namespace {
+
+int32_t bfail(nv_batch *b, int32_t code, const std::string &msg) {
+    if (b) b->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define NV_BHIP(b, call)                                                                            \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return bfail((b), NV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Tile table for bs rows per tile: problem-major, so each problem's tiles are
+// contiguous and its partial sums add in tile order.
+hipError_t batch_tiles(nv_batch *b, int bs) {
+    std::vector<nv::BatchTile> t;
+    std::vector<int32_t> t0;
+    for (size_t p = 0; p < b->ctx.size(); ++p) {
+        t0.push_back((int32_t)t.size());
+        const int64_t D = b->ctx[p]->op.D;
+        for (int64_t r = 0; r < D; r += bs) t.push_back({(int32_t)p, (int32_t)r});
+    }
+    t0.push_back((int32_t)t.size());
+    if (b->tiles_d) hipFree(b->tiles_d);
+    if (b->tile0_d) hipFree(b->tile0_d);
+    b->tiles_d = nullptr;
+    b->tile0_d = nullptr;
+    hipError_t e = hipMalloc(&b->tiles_d, t.size() * sizeof(nv::BatchTile));
+    if (e == hipSuccess) e = hipMalloc(&b->tile0_d, t0.size() * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemcpy(b->tiles_d, t.data(), t.size() * sizeof(nv::BatchTile), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->tile0_d, t0.data(), t0.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    b->d.tiles = b->tiles_d;
+    b->d.tile0 = b->tile0_d;
+    b->d.T = (int32_t)t.size();
+    b->d.bs = bs;
+    b->bs = bs;
+    if (b->graph) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return e;
+}
+
+hipError_t batch_enqueue(nv_batch *b, bool absolute, int64_t it) {
+    hipStream_t s = b->stream;
+    hipError_t e = nv::launch_set_iter(b->iter_d, absolute ? it : 0, absolute ? 0 : 1, s);
+    if (e == hipSuccess) e = nv::launch_batch_normalize(b->d, b->prec, s);
+    if (e == hipSuccess) e = nv::launch_batch_hv(b->d, b->prec, b->lds_hv, s);
+    if (e == hipSuccess) e = nv::launch_batch_step(b->d, b->prec, b->lds_step, s);
+    return e;
+}
+
+void batch_release_run(nv_batch *b) {
+    if (b->graph) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    for (void *p : b->run_allocs) hipFree(p);
+    b->run_allocs.clear();
+    b->ready = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t nv_batch_create(const nv_problem *probs, int64_t count, int32_t device, nv_batch **out) {
+    if (!out) return bfail(nullptr, NV_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!probs || count < 1) return bfail(nullptr, NV_ERR_ARG, "no problems");
+    if (count > (int64_t)1 << 20) return bfail(nullptr, NV_ERR_ARG, "too many problems");
+    nv_batch *b = new (std::nothrow) nv_batch();
+    if (!b) return bfail(nullptr, NV_ERR_NOMEM, "host allocation failed");
+    b->device = device;
+    for (int64_t i = 0; i < count; ++i) {
+        nv_ctx *c = nullptr;
+        int32_t rc = nv_create(&probs[i], device, &c);
+        if (rc != NV_OK) {
+            const std::string msg = "problem " + std::to_string(i) + ": " + g_create_error;
+            nv_batch_destroy(b);
+            return bfail(nullptr, rc, msg);
+        }
+        c->jit_on = false;  // the batch runs the table-driven kernels
+        b->ctx.push_back(c);
+    }
+    std::vector<NvOpDev> ops;
+    for (nv_ctx *c : b->ctx) {
+        b->voff.push_back(b->Ntot);
+        b->loff.push_back(b->Ltot);
+        b->Ntot += c->op.N;
+        b->Ltot += c->op.D * nv_lo_stride((int)c->op.ns);
+        ops.push_back(c->dev);
+        b->lds_hv = std::max(b->lds_hv, nv::hv_lds_bytes(c->dev, NV_PREC_F64));
+        b->lds_step = std::max(b->lds_step, (size_t)c->op.M * nv::kBlock);
+    }
+    auto fail_out = [&](int32_t rc) {
+        const std::string msg = b->err;
+        nv_batch_destroy(b);
+        return bfail(nullptr, rc, msg);
+    };
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&b->ops_d, ops.size() * sizeof(NvOpDev));
+    if (e == hipSuccess) e = hipMalloc(&b->voff_d, b->voff.size() * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->loff_d, b->loff.size() * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->iter_d, sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemcpy(b->ops_d, ops.data(), ops.size() * sizeof(NvOpDev), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->voff_d, b->voff.data(), b->voff.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->loff_d, b->loff.data(), b->loff.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = batch_tiles(b, b->bs);
+    if (e != hipSuccess) {
+        b->err = std::string("batch setup: ") + hipGetErrorString(e);
+        return fail_out(NV_ERR_HIP);
+    }
+    b->d.ops = b->ops_d;
+    b->d.voff = b->voff_d;
+    b->d.loff = b->loff_d;
+    b->d.iter = b->iter_d;
+    b->d.P = (int32_t)b->ctx.size();
+    b->d.Ntot = b->Ntot;
+    *out = b;
+    return NV_OK;
+}
+
+int32_t nv_batch_destroy(nv_batch *b) {
+    if (!b) return NV_OK;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    batch_release_run(b);
+    for (void *p : {(void *)b->ops_d, (void *)b->voff_d, (void *)b->loff_d, (void *)b->iter_d, (void *)b->tiles_d,
+                    (void *)b->tile0_d})
+        if (p) hipFree(p);
+    for (nv_ctx *c : b->ctx) nv_destroy(c);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return NV_OK;
+}
+
+const char *nv_batch_last_error(const nv_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int32_t nv_batch_init(nv_batch *b, int64_t niter, int32_t prec) {
+    if (!b) return NV_ERR_ARG;
+    if (niter < 1) return bfail(b, NV_ERR_ARG, "niter must be >= 1");
+    if (prec != NV_PREC_F64 && prec != NV_PREC_F32) return bfail(b, NV_ERR_ARG, "batch runs support F64 and F32");
+    NV_BHIP(b, hipSetDevice(b->device));
+    NV_BHIP(b, hipStreamSynchronize(b->stream));
+    batch_release_run(b);
+    const int64_t P = (int64_t)b->ctx.size(), ld = niter + 2;
+    int64_t maxtiles = 0;
+    for (nv_ctx *c : b->ctx) maxtiles += (c->op.D + 63) / 64;  // the smallest tile size
+    auto alloc = [&](size_t bytes, void **p) -> hipError_t {
+        hipError_t er = hipMalloc(p, bytes);
+        if (er == hipSuccess) b->run_allocs.push_back(*p);
+        return er;
+    };
+    double *V = nullptr, *al = nullptr, *be = nullptr, *parts = nullptr;
+    void *vlo = nullptr, *y = nullptr;
+    NV_BHIP(b, alloc((size_t)3 * b->Ntot * sizeof(double), (void **)&V));
+    NV_BHIP(b, alloc((size_t)b->Ntot * (prec == NV_PREC_F64 ? 8 : 4), &y));
+    if (prec == NV_PREC_F32) {
+        NV_BHIP(b, alloc((size_t)b->Ltot * sizeof(float), &vlo));
+        NV_BHIP(b, hipMemset(vlo, 0, (size_t)b->Ltot * sizeof(float)));
+    }
+    NV_BHIP(b, alloc((size_t)P * ld * sizeof(double), (void **)&al));
+    NV_BHIP(b, alloc((size_t)P * ld * sizeof(double), (void **)&be));
+    NV_BHIP(b, alloc((size_t)maxtiles * sizeof(double), (void **)&parts));
+    NV_BHIP(b, hipMemset(V, 0, (size_t)3 * b->Ntot * sizeof(double)));
+    NV_BHIP(b, hipMemset(al, 0, (size_t)P * ld * sizeof(double)));
+    NV_BHIP(b, hipMemset(be, 0, (size_t)P * ld * sizeof(double)));
+    // start vectors (each problem's own: default or Franck-Condon) into slot 1, beta(0) = |start|
+    std::vector<double> beta0(P * ld, 0.0);
+    for (int64_t p = 0; p < P; ++p) {
+        nv_ctx *c = b->ctx[p];
+        std::vector<double> q(c->op.N);
+        int32_t rc = nv_start_vector(c, q.data());
+        if (rc != NV_OK) return bfail(b, rc, "problem " + std::to_string(p) + ": " + c->err);
+        double nn = 0.0;
+        for (double v : q) nn += v * v;
+        beta0[p * ld] = std::sqrt(nn);
+        NV_BHIP(b, hipMemcpy(V + b->Ntot + b->voff[p], q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    NV_BHIP(b, hipMemcpy(be, beta0.data(), beta0.size() * sizeof(double), hipMemcpyHostToDevice));
+    b->d.V = V;
+    b->d.vlo = vlo;
+    b->d.y = y;
+    b->d.alpha = al;
+    b->d.beta = be;
+    b->d.ld = ld;
+    b->d.partials = parts;
+    b->prec = prec;
+    b->niter = niter;
+    b->last = 0;
+    b->next = 1;
+    b->lds_hv = 0;
+    for (nv_ctx *c : b->ctx) b->lds_hv = std::max(b->lds_hv, nv::hv_lds_bytes(c->dev, prec));
+    b->ready = true;
+    return NV_OK;
+}
+
+int32_t nv_batch_run(nv_batch *b, int64_t iiter, int64_t niter) {
+    if (!b) return NV_ERR_ARG;
+    if (!b->ready) return bfail(b, NV_ERR_STATE, "nv_batch_init has not been called");
+    if (iiter != b->next) return bfail(b, NV_ERR_STATE, "run must start at the next iteration");
+    if (niter > b->niter) return bfail(b, NV_ERR_ARG, "niter beyond the initialised run");
+    NV_BHIP(b, hipSetDevice(b->device));
+    for (int64_t i = iiter; i <= niter; ++i) {
+        if (i == iiter) {
+            NV_BHIP(b, batch_enqueue(b, true, i));
+        } else {
+            if (!b->graph) {
+                hipGraph_t g = nullptr;
+                NV_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                hipError_t e2 = batch_enqueue(b, false, 0);
+                hipError_t e = hipStreamEndCapture(b->stream, &g);
+                if (e2 != hipSuccess) e = e2;
+                if (e == hipSuccess) e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+                if (g) hipGraphDestroy(g);
+                if (e != hipSuccess) return bfail(b, NV_ERR_HIP, std::string("batch graph: ") + hipGetErrorString(e));
+            }
+            NV_BHIP(b, hipGraphLaunch(b->graph, b->stream));
+        }
+        b->last = i;
+        b->next = i + 1;
+    }
+    return NV_OK;
+}
+
+int32_t nv_batch_coefficients(nv_batch *b, int64_t p, double *alpha, double *beta) {
+    if (!b) return NV_ERR_ARG;
+    if (!b->ready) return bfail(b, NV_ERR_STATE, "no batch run");
+    if (p < 0 || p >= (int64_t)b->ctx.size()) return bfail(b, NV_ERR_ARG, "problem index out of range");
+    NV_BHIP(b, hipSetDevice(b->device));
+    NV_BHIP(b, hipStreamSynchronize(b->stream));
+    const int64_t n = b->last;
+    if (alpha && n > 0)
+        NV_BHIP(b, hipMemcpy(alpha, b->d.alpha + p * b->d.ld + 1, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (beta) NV_BHIP(b, hipMemcpy(beta, b->d.beta + p * b->d.ld, (n + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    return NV_OK;
+}
+
+nv_ctx *nv_batch_context(nv_batch *b, int64_t p) {
+    if (!b || p < 0 || p >= (int64_t)b->ctx.size()) return nullptr;
+    return b->ctx[p];
+}
+
+int32_t nv_batch_tune(nv_batch *b, char *report, int64_t len) {
+    if (!b) return NV_ERR_ARG;
+    if (!b->ready || b->last != 0) return bfail(b, NV_ERR_STATE, "tune after nv_batch_init, before the run");
+    NV_BHIP(b, hipSetDevice(b->device));
+    const int cand[3] = {64, 128, 256};
+    float best_ms = 0.0f;
+    int best = b->bs;
+    std::string rep = "batched H*v, rows per workgroup:";
+    hipEvent_t e0, e1;
+    NV_BHIP(b, hipEventCreate(&e0));
+    NV_BHIP(b, hipEventCreate(&e1));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+        e = batch_tiles(b, cand[k]);
+        if (e == hipSuccess) e = nv::launch_set_iter(b->iter_d, 1, 0, b->stream);
+        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = nv::launch_batch_hv(b->d, b->prec, b->lds_hv, b->stream);
+        if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+        for (int w = 0; w < 10 && e == hipSuccess; ++w) e = nv::launch_batch_hv(b->d, b->prec, b->lds_hv, b->stream);
+        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        ms /= 10.0f;
+        char buf[64];
+        std::snprintf(buf, sizeof(buf), " %d: %.1f us;", cand[k], 1000.0 * ms);
+        rep += buf;
+        if (k == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = cand[k];
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (e == hipSuccess) e = batch_tiles(b, best);
+    // alpha(1) of the timing launches is recomputed by the first iteration
+    if (e == hipSuccess) e = hipMemsetAsync(b->d.alpha, 0, (size_t)b->ctx.size() * b->d.ld * sizeof(double), b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return bfail(b, NV_ERR_HIP, std::string("batch tune: ") + hipGetErrorString(e));
+    rep += " chosen " + std::to_string(best);
+    if (report && len > 0) std::snprintf(report, (size_t)len, "%s", rep.c_str());
     return NV_OK;
 }
 

@@ -210,8 +210,10 @@ __device__ __forceinline__ double nv_block_sum(double v) {
         return o.str();
     };
     // iteration kernel: x = vlo (F32) or q (F64); y = off-diagonal part; alpha partials
-    s << "extern \"C\" __global__ __launch_bounds__(256) void nv_hv_iter(const double *V, long long nslots, const T "
-         "*vlo, T *y, const double *dg, double *partials, const long long *iter) {\n";
+    const char *sfx = f32 ? "_f32" : "_f64";
+    s << "extern \"C\" __global__ __launch_bounds__(256) void nv_hv_iter" << sfx
+      << "(const double *V, long long nslots, const T *vlo, T *y, const double *dg, double *partials, "
+         "const long long *iter) {\n";
     s << "    const long long it = *iter;\n";
     s << "    const double *q = V + (it % nslots) * NV_N;\n";
     s << (f32 ? "    const T *x = vlo;\n" : "    const T *x = q;\n");
@@ -227,8 +229,8 @@ __device__ __forceinline__ double nv_block_sum(double v) {
     s << "    }\n    const double bsum = nv_block_sum(part);\n";
     s << "    if (threadIdx.x == 0) partials[blockIdx.x] = bsum;\n}\n";
     // standalone kernel: x double in, y = H x double out
-    s << "extern \"C\" __global__ __launch_bounds__(256) void nv_hv_full(const double *xd, const T *x, double *yd, "
-         "const double *dg) {\n";
+    s << "extern \"C\" __global__ __launch_bounds__(256) void nv_hv_full" << sfx
+      << "(const double *xd, const T *x, double *yd, const double *dg) {\n";
     s << b.str();
     s << "    if (live) {\n";
     for (int q = 0; q < ns; ++q) s << "        const double v" << q << " = xd[" << q << "LL * NV_D + r];\n";
@@ -274,8 +276,9 @@ hipError_t jit_build(const Operator &op, int prec, JitKernels &out, std::string 
     hiprtcDestroyProgram(&prog);
     hipError_t e = hipModuleLoadData(&out.module, code.data());
     if (e != hipSuccess) return e;
-    e = hipModuleGetFunction(&out.iter, out.module, "nv_hv_iter");
-    if (e == hipSuccess) e = hipModuleGetFunction(&out.full, out.module, "nv_hv_full");
+    const std::string sfx = prec != 0 ? "_f32" : "_f64";
+    e = hipModuleGetFunction(&out.iter, out.module, ("nv_hv_iter" + sfx).c_str());
+    if (e == hipSuccess) e = hipModuleGetFunction(&out.full, out.module, ("nv_hv_full" + sfx).c_str());
     out.ready = e == hipSuccess;
     return e;
 }

@@ -460,6 +460,10 @@ static hipError_t step_any(const NvOpDev &op, int prec, const LzState &st, hipSt
     NV_DISPATCH_NS(step_ns, op, prec, st, s)
 }
 
+hipError_t launch_hv_iter_only(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    return hv_iter_any(op, prec, st, s);
+}
+
 hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     hipError_t e = hv_iter_any(op, prec, st, s);
     if (e != hipSuccess) return e;
@@ -796,6 +800,178 @@ hipError_t launch_tridiag(int64_t n, const TriWork &w, double anorm, hipStream_t
     hipLaunchKernelGGL(tri_cluster_kernel, dim3(1), dim3(64), 0, s, n, w.root, w.ia, w.rank);
     hipLaunchKernelGGL(tri_inverse_kernel, dim3(g), dim3(tb), 0, s, n, w.d, w.e, w.root, w.ia, w.rank, w.lu, w.vec);
     hipLaunchKernelGGL(tri_finish_kernel, dim3((unsigned)n), dim3(256), 0, s, n, w.ia, w.vec);
+    return hipGetLastError();
+}
+
+
+// ------------------------------------------------------------------------
+// Batched Lanczos: many operators of uneven size advance one iteration per
+// launch sequence.  The rows of every problem are cut into tiles of
+// blockDim.x rows; one workgroup per tile, so one launch covers the whole
+// batch and the small problems cost no launches of their own.  Each tile
+// reads its problem's operator tables and vector offsets.
+// ------------------------------------------------------------------------
+template <int NS, typename TIn, typename TAcc>
+__device__ __forceinline__ void batch_hv_row(const NvOpDev &op, const TIn *x, const double *q, TAcc *y, int r,
+                                             const uint8_t *dig, const TAcc *ftab, int tid, double &part) {
+    TAcc acc[NS];
+    offdiag_accumulate<NS, TIn, TAcc>(op, x, r, dig, ftab, tid, acc);
+    double v[NS], dv[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
+    diag_apply<NS>(op, r, dig, tid, v, dv);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        y[(size_t)s * op.D + r] = acc[s];
+        part += v[s] * (dv[s] + (double)acc[s]);
+    }
+}
+
+#define NV_BATCH_NS(call)                                                                                         \
+    switch (op.ns) {                                                                                              \
+    case 1: call(1); break;                                                                                       \
+    case 2: call(2); break;                                                                                       \
+    case 3: call(3); break;                                                                                       \
+    case 4: call(4); break;                                                                                       \
+    case 5: call(5); break;                                                                                       \
+    case 6: call(6); break;                                                                                       \
+    case 7: call(7); break;                                                                                       \
+    case 8: call(8); break;                                                                                       \
+    case 9: call(9); break;                                                                                       \
+    default: call(10); break;                                                                                     \
+    }
+
+template <typename TIn>
+__global__ __launch_bounds__(kBlock) void batch_normalize_kernel(BatchDev b) {
+    const BatchTile tl = b.tiles[blockIdx.x];
+    const NvOpDev &opg = b.ops[tl.prob];
+    const int D = opg.D, ns = opg.ns;
+    const int r = tl.row0 + threadIdx.x;
+    if (r >= D) return;
+    const int64_t it = *b.iter;
+    double *q = b.V + slot_of(it, 3) * b.Ntot + b.voff[tl.prob];
+    const double sc = 1.0 / b.beta[(int64_t)tl.prob * b.ld + it - 1];
+    TIn *lo = b.vlo ? reinterpret_cast<TIn *>(b.vlo) + b.loff[tl.prob] : nullptr;
+    const int nsp = nv_lo_stride(ns);
+    for (int s = 0; s < ns; ++s) {
+        const double v = q[(size_t)s * D + r] * sc;
+        q[(size_t)s * D + r] = v;
+        if (lo) lo[(size_t)r * nsp + s] = from_d<TIn>(v);
+    }
+}
+
+template <typename TIn, typename TAcc>
+__global__ __launch_bounds__(kBlock) void batch_hv_kernel(BatchDev b) {
+    const BatchTile tl = b.tiles[blockIdx.x];
+    const NvOpDev op = b.ops[tl.prob];
+    uint8_t *dig;
+    const TAcc *ftab = stage_tables<TAcc>(op, dig);
+    const int tid = threadIdx.x;
+    const int r = tl.row0 + tid;
+    const bool live = r < op.D;
+    const int64_t it = *b.iter;
+    const double *q = b.V + slot_of(it, 3) * b.Ntot + b.voff[tl.prob];
+    const TIn *x = b.vlo ? reinterpret_cast<const TIn *>(b.vlo) + b.loff[tl.prob] : reinterpret_cast<const TIn *>(q);
+    TAcc *y = reinterpret_cast<TAcc *>(b.y) + b.voff[tl.prob];
+    if (live) digits_energy(op, r, dig, tid);
+    __syncthreads();
+    double part = 0.0;
+    if (live) {
+#define NV_HV_ROW(n) batch_hv_row<n, TIn, TAcc>(op, x, q, y, r, dig, ftab, tid, part)
+        NV_BATCH_NS(NV_HV_ROW)
+#undef NV_HV_ROW
+    }
+    const double bs = block_sum(part);
+    if (tid == 0) b.partials[blockIdx.x] = bs;
+}
+
+template <int NS, typename TAcc>
+__device__ __forceinline__ void batch_step_row(const NvOpDev &op, const double *q, const double *qm, double *res,
+                                               const TAcc *y, double a, double bb, int64_t it, int r,
+                                               const uint8_t *dig, int tid, double &part) {
+    double v[NS], dv[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
+    diag_apply<NS>(op, r, dig, tid, v, dv);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const size_t o = (size_t)s * op.D + r;
+        double w = dv[s] + (double)y[o];
+        w = w - a * v[s];
+        if (it > 1) w = w - bb * qm[o];
+        res[o] = w;
+        part += w * w;
+    }
+}
+
+template <typename TAcc>
+__global__ __launch_bounds__(kBlock) void batch_step_kernel(BatchDev b) {
+    extern __shared__ __align__(16) unsigned char dig[];
+    const BatchTile tl = b.tiles[blockIdx.x];
+    const NvOpDev op = b.ops[tl.prob];
+    const int tid = threadIdx.x;
+    const int r = tl.row0 + tid;
+    const int64_t it = *b.iter;
+    const int64_t off = b.voff[tl.prob];
+    const double *q = b.V + slot_of(it, 3) * b.Ntot + off;
+    const double *qm = b.V + slot_of(it - 1, 3) * b.Ntot + off;
+    double *res = b.V + slot_of(it + 1, 3) * b.Ntot + off;
+    const TAcc *y = reinterpret_cast<const TAcc *>(b.y) + off;
+    const double a = b.alpha[(int64_t)tl.prob * b.ld + it];
+    const double bb = it > 1 ? b.beta[(int64_t)tl.prob * b.ld + it - 1] : 0.0;
+    double part = 0.0;
+    if (r < op.D) {
+        if (op.offstate) digits_energy(op, r, dig, tid);
+#define NV_STEP_ROW(n) batch_step_row<n, TAcc>(op, q, qm, res, y, a, bb, it, r, dig, tid, part)
+        NV_BATCH_NS(NV_STEP_ROW)
+#undef NV_STEP_ROW
+    }
+    const double bs = block_sum(part);
+    if (tid == 0) b.partials[blockIdx.x] = bs;
+}
+
+// dst[p][it] = (sqrt of) the sum of problem p's tile partials, in tile order.
+__global__ __launch_bounds__(256) void batch_finish_kernel(BatchDev b, double *dst, int take_sqrt) {
+    const int p = blockIdx.x;
+    const int t0 = b.tile0[p], t1 = b.tile0[p + 1];
+    double s = 0.0;
+    for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) s += b.partials[t];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        const int64_t it = *b.iter;
+        dst[(int64_t)p * b.ld + it] = take_sqrt ? sqrt(s) : s;
+    }
+}
+
+size_t batch_hv_lds(const NvOpDev &op, int prec) { return hv_lds_bytes(op, prec); }
+
+hipError_t launch_batch_normalize(const BatchDev &b, int prec, hipStream_t s) {
+    if (prec == 0 || b.vlo == nullptr)
+        hipLaunchKernelGGL((batch_normalize_kernel<double>), dim3(b.T), dim3(b.bs), 0, s, b);
+    else
+        hipLaunchKernelGGL((batch_normalize_kernel<float>), dim3(b.T), dim3(b.bs), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_hv(const BatchDev &b, int prec, size_t lds, hipStream_t s) {
+    if (prec == 0)
+        hipLaunchKernelGGL((batch_hv_kernel<double, double>), dim3(b.T), dim3(b.bs), lds, s, b);
+    else
+        hipLaunchKernelGGL((batch_hv_kernel<float, float>), dim3(b.T), dim3(b.bs), lds, s, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_finish_kernel, dim3(b.P), dim3(256), 0, s, b, b.alpha, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step(const BatchDev &b, int prec, size_t lds, hipStream_t s) {
+    if (prec == 0)
+        hipLaunchKernelGGL((batch_step_kernel<double>), dim3(b.T), dim3(b.bs), lds, s, b);
+    else
+        hipLaunchKernelGGL((batch_step_kernel<float>), dim3(b.T), dim3(b.bs), lds, s, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_finish_kernel, dim3(b.P), dim3(256), 0, s, b, b.beta, 1);
     return hipGetLastError();
 }
 

@@ -43,6 +43,8 @@ hipError_t launch_set_iter(int64_t *d_iter, int64_t value, int increment, hipStr
 hipError_t launch_normalize(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
 // Hv: off-diagonal part of H q_i and alpha(i) = q_i . H q_i.
 hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
+// The H*v kernel of launch_hv_iter alone (its alpha partials are not summed).
+hipError_t launch_hv_iter_only(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
 // lanczos_step: r_i = H q_i - alpha q_i - beta(i-1) q_{i-1}, beta(i) = |r_i|.
 hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
 // Writes the start vector: zero except vib index 0 of each state, from vals[ns].
@@ -89,6 +91,33 @@ hipError_t launch_topk(const double *Y, int64_t N, int nroots, int kmax, double 
 void gram_shape(int64_t D, int R, int &npairs, int &nchunks, int64_t &rows_per_chunk);
 // G (R x R) = A^T A, column c of A at A + c*D.
 hipError_t launch_gram(const double *A, int64_t D, int R, double *part, double *G, hipStream_t s);
+
+// ---- batched Lanczos (batch_* kernels in nv_kernels.hip) ----
+struct BatchTile {
+    int32_t prob;  // problem index
+    int32_t row0;  // first vibrational row of the tile
+};
+struct BatchDev {
+    const NvOpDev *ops = nullptr;     // [P] operator tables
+    const BatchTile *tiles = nullptr; // [T] tiles, problem-major
+    const int32_t *tile0 = nullptr;   // [P+1] first tile of each problem
+    const int64_t *voff = nullptr;    // [P] offset of problem p in the state-major vectors
+    const int64_t *loff = nullptr;    // [P] offset of problem p in the interleaved low-precision copy
+    double *V = nullptr;              // 3 slots of Ntot doubles (slot i % 3)
+    int64_t Ntot = 0;
+    void *vlo = nullptr;              // interleaved FP32 copy (mixed precision)
+    void *y = nullptr;                // off-diagonal part of H q_i, state-major
+    double *alpha = nullptr, *beta = nullptr;  // [P][ld], alpha[p][i], beta[p][i]
+    int64_t ld = 0;
+    double *partials = nullptr;       // [T]
+    const int64_t *iter = nullptr;    // current iteration (device)
+    int32_t P = 0, T = 0, bs = 256;   // problems, tiles, rows per tile (= workgroup size)
+};
+size_t batch_hv_lds(const NvOpDev &op, int prec);
+// update_lanczos, Hv (+ alpha finish) and lanczos_step (+ beta finish) for every problem.
+hipError_t launch_batch_normalize(const BatchDev &b, int prec, hipStream_t s);
+hipError_t launch_batch_hv(const BatchDev &b, int prec, size_t lds, hipStream_t s);
+hipError_t launch_batch_step(const BatchDev &b, int prec, size_t lds, hipStream_t s);
 
 // ---- adjoint (nv_adjoint.hip) ----
 int adj_blocks(int64_t N);

@@ -19,31 +19,35 @@ from ._lib import NvError, dptr, iptr
 from .inputs import AU2EV, AU2WAVE, Problem
 
 
+def problem_struct(p: Problem):
+    """The nv_problem of p and the arrays it points into (keep them alive)."""
+    nfunc = np.ascontiguousarray(p.nfunc, dtype=np.int64)
+    aomega = np.ascontiguousarray(p.aomega, dtype=np.float64)
+    concoef = np.ascontiguousarray(p.concoef, dtype=np.float64)
+    pot = np.ascontiguousarray(p.potterms, dtype=np.float64)
+    statew = np.ascontiguousarray(p.statew if p.statew is not None else np.ones(p.nstates), dtype=np.float64)
+    npirr = np.ascontiguousarray(p.npirrep or [p.nmodes], dtype=np.int64)
+    istate = np.ascontiguousarray(p.initstate or [0] * p.nmodes, dtype=np.int64)
+    neworigin = bool(p.shiftref)
+    bo = dv = tm = None
+    if neworigin:
+        bo = np.ascontiguousarray(p.bomega, dtype=np.float64)
+        dv = np.ascontiguousarray(p.dvec, dtype=np.float64)
+        tm = np.ascontiguousarray(p.tmat, dtype=np.float64)
+    keep = [nfunc, aomega, concoef, pot, statew, npirr, istate, bo, dv, tm]
+    desc = _lib.NvProblem(
+        nstates=p.nstates, nmodes=p.nmodes, ordr=p.ordr, nfunc=iptr(nfunc), aomega=dptr(aomega),
+        concoef=dptr(concoef), potterms=dptr(pot), ztoler=p.ztoler, quirks=p.quirks, neworigin=int(neworigin),
+        statew=dptr(statew), bomega=dptr(bo), dvec=dptr(dv), tmat=dptr(tm), nirreps=len(npirr),
+        npirrep=iptr(npirr), istate=iptr(istate))
+    return desc, keep
+
+
 class Engine:
     def __init__(self, prob: Problem, device=0):
         self.lib = _lib.load()
         self.prob = prob
-        p = prob
-        self._keep = []
-        nfunc = np.ascontiguousarray(p.nfunc, dtype=np.int64)
-        aomega = np.ascontiguousarray(p.aomega, dtype=np.float64)
-        concoef = np.ascontiguousarray(p.concoef, dtype=np.float64)
-        pot = np.ascontiguousarray(p.potterms, dtype=np.float64)
-        statew = np.ascontiguousarray(p.statew if p.statew is not None else np.ones(p.nstates), dtype=np.float64)
-        npirr = np.ascontiguousarray(p.npirrep or [p.nmodes], dtype=np.int64)
-        istate = np.ascontiguousarray(p.initstate or [0] * p.nmodes, dtype=np.int64)
-        neworigin = bool(p.shiftref)
-        bo = dv = tm = None
-        if neworigin:
-            bo = np.ascontiguousarray(p.bomega, dtype=np.float64)
-            dv = np.ascontiguousarray(p.dvec, dtype=np.float64)
-            tm = np.ascontiguousarray(p.tmat, dtype=np.float64)
-        self._keep += [nfunc, aomega, concoef, pot, statew, npirr, istate, bo, dv, tm]
-        desc = _lib.NvProblem(
-            nstates=p.nstates, nmodes=p.nmodes, ordr=p.ordr, nfunc=iptr(nfunc), aomega=dptr(aomega),
-            concoef=dptr(concoef), potterms=dptr(pot), ztoler=p.ztoler, quirks=p.quirks, neworigin=int(neworigin),
-            statew=dptr(statew), bomega=dptr(bo), dvec=dptr(dv), tmat=dptr(tm), nirreps=len(npirr),
-            npirrep=iptr(npirr), istate=iptr(istate))
+        desc, self._keep = problem_struct(prob)
         h = ctypes.c_void_p()
         rc = self.lib.nv_create(ctypes.byref(desc), device, ctypes.byref(h))
         if rc != 0:
@@ -263,6 +267,12 @@ class Engine:
 
     def profile(self, on=True):
         self._check(self.lib.nv_profile_enable(self.h, 1 if on else 0))
+
+    def profile_hv(self, launches=200):
+        """Average microseconds of back-to-back iteration H*v launches (after update_lanczos(1))."""
+        us = ctypes.c_double()
+        self._check(self.lib.nv_profile_hv(self.h, launches, ctypes.byref(us)))
+        return us.value
 
     def profile_read(self):
         """(milliseconds, launches) of the timed H*v launches since the last read."""

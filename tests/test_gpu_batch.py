@@ -1,0 +1,78 @@
+"""GPU: batched runs (nv_batch_*) against single-problem runs and the oracle.
+
+Every problem of a batch must reproduce its own single-problem run: the same
+per-row H*v arithmetic (table-driven kernel) and the same recurrence, with the
+dots summed over different partitions, so alpha/beta agree to 1e-11
+relative over 40 iterations in FP64 (1e-8 in FP32); against the oracle the
+first 20 coefficients agree to 1e-10.
+"""
+import numpy as np
+import pytest
+
+from nadvibs_amd.batch import Batch
+from nadvibs_amd.engine import Engine
+from nadvibs_amd.synth import batch_workloads, make_problem
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, [7, 5]), (2, [4, 3, 5]), (3, [3, 4, 2, 3]), (2, [6, 6, 5]), (3, [5, 4, 4, 3, 2]), (1, [9])]
+
+
+def _probs():
+    return [make_problem(ns, nf, 2, (500, 3000), 250.0, 60.0, 40.0, seed=50 + i) for i, (ns, nf) in enumerate(SHAPES)]
+
+
+def _single(p, n, prec):
+    with Engine(p) as eng:
+        eng.select_hv_kernel(True)
+        eng.lanczos_init(n, prec=prec)
+        eng.run(1, n)
+        return eng.coefficients()[:2]
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-11), ("f32", 1e-8)])
+def test_batch_matches_single_runs(prec, tol):
+    probs = _probs()
+    n = 40
+    with Batch(probs) as bt:
+        bt.init(n, prec)
+        bt.run()
+        got = [bt.coefficients(i) for i in range(len(probs))]
+    for p, (a, b) in zip(probs, got):
+        a1, b1 = _single(p, n, prec)
+        np.testing.assert_allclose(a, a1, rtol=tol, atol=1e-15)
+        np.testing.assert_allclose(b, b1, rtol=tol, atol=1e-15)
+
+
+def test_batch_matches_oracle_and_tuning_keeps_results():
+    probs = _probs()
+    n = 30
+    with Batch(probs) as bt:
+        bt.init(n, "f64")
+        rep = bt.tune()
+        assert "chosen" in rep
+        bt.run(1, 10)
+        bt.run(11, n)
+        res = [bt.coefficients(i) for i in range(len(probs))]
+        ev, it, bj, _ = bt.ritz(2, *res[2])
+    for p, (a, b) in zip(probs, res):
+        r = Oracle(p).lanczos(n)
+        np.testing.assert_allclose(a[:20], r["alpha"][:20], rtol=1e-10)
+        np.testing.assert_allclose(b[:20], r["beta"][:20], rtol=1e-10)
+    assert np.all(np.diff(ev) >= 0) and abs(np.sum(it) - 1.0) < 1e-10
+
+
+def test_batch_uneven_workload_sample():
+    """The 64-problem uneven batch: three problems against single runs."""
+    probs = batch_workloads()
+    n = 12
+    with Batch(probs) as bt:
+        bt.init(n, "f32")
+        bt.run()
+        picks = [0, 17, 63]
+        got = {i: bt.coefficients(i) for i in picks}
+    for i in picks:
+        a1, b1 = _single(probs[i], n, "f32")
+        np.testing.assert_allclose(got[i][0], a1, rtol=1e-8)
+        np.testing.assert_allclose(got[i][1], b1, rtol=1e-8)
