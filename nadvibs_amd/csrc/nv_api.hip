@@ -1346,6 +1346,11 @@ int32_t bfail(nv_batch *b, int32_t code, const std::string &msg) {
 // Tile table for bs rows per tile: problem-major, so each problem's tiles are
 // contiguous and its partial sums add in tile order.
 hipError_t batch_tiles(nv_batch *b, int bs) {
+    // launches already queued may still read the current tables
+    if (b->stream) {
+        hipError_t e0 = hipStreamSynchronize(b->stream);
+        if (e0 != hipSuccess) return e0;
+    }
     std::vector<nv::BatchTile> t;
     std::vector<int32_t> t0;
     for (size_t p = 0; p < b->ctx.size(); ++p) {
