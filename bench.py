@@ -117,18 +117,30 @@ def batched_cfg5(sync, cpu_seconds):
         e = Engine(p)
         e.select_hv_kernel(True)
         e.lanczos_init(niter, prec="f32")
-        engs.append(e)
-    sync()
-    t0 = time.perf_counter()
-    for e in engs:
-        e.run(1, niter)
-    for e in engs:
+        e.run(1, 2)   # capture the iteration graph
         e.coefficients()
-    t_seq = time.perf_counter() - t0
+        engs.append(e)
+
+    def one_by_one(wait_each):
+        for e in engs:
+            e.lanczos_init(niter, prec="f32")
+        sync()
+        t0 = time.perf_counter()
+        for e in engs:
+            e.run(1, niter)
+            if wait_each:
+                e.coefficients()
+        for e in engs:
+            e.coefficients()
+        return time.perf_counter() - t0
+
+    t_seq = one_by_one(True)     # one problem after the other
+    t_streams = one_by_one(False)  # every problem on its own stream, all enqueued at once
     a1, _, _ = engs[0].coefficients()
     for e in engs:
         e.close()
-    print(f"# sequential cfg5: {t_seq * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    print(f"# cfg5 one by one: {t_seq * 1e3:.1f} ms, concurrent streams: {t_streams * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
     # host: the oracle over all problems for k iterations, k doubled until the sample lasts cpu_seconds / 3
     from oracle.oracle import Oracle
     k, dt = 1, 0.0
@@ -142,8 +154,8 @@ def batched_cfg5(sync, cpu_seconds):
         k *= 2
     return {"workload": "cfg5: 64 problems, nstates U{2,3}, nmodes U{4..6}, basis U{4..12}, ordr 2, "
                         f"total N={total}", "niter": niter, "hv_precision": "f32",
-            "batched_s": round(t_batch, 4), "sequential_s": round(t_seq, 4),
-            "speedup_vs_sequential": round(t_seq / t_batch, 2),
+            "batched_s": round(t_batch, 4), "sequential_s": round(t_seq, 4), "streams_s": round(t_streams, 4),
+            "speedup_vs_sequential": round(t_seq / t_batch, 2), "speedup_vs_streams": round(t_streams / t_batch, 2),
             "problem_iter_per_s": round(len(probs) * niter / t_batch, 1),
             "alpha1_agree": bool(abs(a0[0] - a1[0]) <= 1e-12 * abs(a1[0])), "tune": rep,
             "cpu_baseline": {"value": round(len(probs) * k / dt, 2), "unit": "problem-iterations/s", "cores": 1,

@@ -1327,6 +1327,7 @@ struct nv_batch {
     size_t lds_hv = 0, lds_step = 0;
     hipGraphExec_t graph = nullptr;
     int bs = 256;
+    std::vector<nv::BatchGroup> groups;
 };
 
 namespace {
@@ -1343,22 +1344,32 @@ int32_t bfail(nv_batch *b, int32_t code, const std::string &msg) {
         if (e_ != hipSuccess) return bfail((b), NV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Tile table for bs rows per tile: problem-major, so each problem's tiles are
-// contiguous and its partial sums add in tile order.
+// Tile table for bs rows per tile: problems grouped by their number of
+// states (one specialised launch per group), each problem's tiles contiguous
+// so that its partial sums add in tile order.
 hipError_t batch_tiles(nv_batch *b, int bs) {
     // launches already queued may still read the current tables
     if (b->stream) {
         hipError_t e0 = hipStreamSynchronize(b->stream);
         if (e0 != hipSuccess) return e0;
     }
+    const int64_t P = (int64_t)b->ctx.size();
+    std::vector<int64_t> order(P);
+    for (int64_t p = 0; p < P; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t x, int64_t y) { return b->ctx[x]->op.ns < b->ctx[y]->op.ns; });
     std::vector<nv::BatchTile> t;
-    std::vector<int32_t> t0;
-    for (size_t p = 0; p < b->ctx.size(); ++p) {
-        t0.push_back((int32_t)t.size());
+    std::vector<int32_t> t0(2 * P);
+    b->groups.clear();
+    for (int64_t p : order) {
+        const int ns = (int)b->ctx[p]->op.ns;
+        if (b->groups.empty() || b->groups.back().ns != ns) b->groups.push_back({ns, (int)t.size(), 0});
+        t0[2 * p] = (int32_t)t.size();
         const int64_t D = b->ctx[p]->op.D;
         for (int64_t r = 0; r < D; r += bs) t.push_back({(int32_t)p, (int32_t)r});
+        t0[2 * p + 1] = (int32_t)t.size();
+        b->groups.back().nt = (int)t.size() - b->groups.back().t0;
     }
-    t0.push_back((int32_t)t.size());
     if (b->tiles_d) hipFree(b->tiles_d);
     if (b->tile0_d) hipFree(b->tile0_d);
     b->tiles_d = nullptr;
@@ -1383,8 +1394,9 @@ hipError_t batch_enqueue(nv_batch *b, bool absolute, int64_t it) {
     hipStream_t s = b->stream;
     hipError_t e = nv::launch_set_iter(b->iter_d, absolute ? it : 0, absolute ? 0 : 1, s);
     if (e == hipSuccess) e = nv::launch_batch_normalize(b->d, b->prec, s);
-    if (e == hipSuccess) e = nv::launch_batch_hv(b->d, b->prec, b->lds_hv, s);
-    if (e == hipSuccess) e = nv::launch_batch_step(b->d, b->prec, b->lds_step, s);
+    if (e == hipSuccess) e = nv::launch_batch_hv(b->d, b->groups.data(), (int)b->groups.size(), b->prec, b->lds_hv, s);
+    if (e == hipSuccess)
+        e = nv::launch_batch_step(b->d, b->groups.data(), (int)b->groups.size(), b->prec, b->lds_step, s);
     return e;
 }
 
@@ -1598,9 +1610,12 @@ int32_t nv_batch_tune(nv_batch *b, char *report, int64_t len) {
     for (int k = 0; k < 3 && e == hipSuccess; ++k) {
         e = batch_tiles(b, cand[k]);
         if (e == hipSuccess) e = nv::launch_set_iter(b->iter_d, 1, 0, b->stream);
-        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = nv::launch_batch_hv(b->d, b->prec, b->lds_hv, b->stream);
+        auto hv = [&]() {
+            return nv::launch_batch_hv(b->d, b->groups.data(), (int)b->groups.size(), b->prec, b->lds_hv, b->stream);
+        };
+        for (int w = 0; w < 2 && e == hipSuccess; ++w) e = hv();
         if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-        for (int w = 0; w < 10 && e == hipSuccess; ++w) e = nv::launch_batch_hv(b->d, b->prec, b->lds_hv, b->stream);
+        for (int w = 0; w < 10 && e == hipSuccess; ++w) e = hv();
         if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0.0f;

@@ -827,20 +827,6 @@ __device__ __forceinline__ void batch_hv_row(const NvOpDev &op, const TIn *x, co
     }
 }
 
-#define NV_BATCH_NS(call)                                                                                         \
-    switch (op.ns) {                                                                                              \
-    case 1: call(1); break;                                                                                       \
-    case 2: call(2); break;                                                                                       \
-    case 3: call(3); break;                                                                                       \
-    case 4: call(4); break;                                                                                       \
-    case 5: call(5); break;                                                                                       \
-    case 6: call(6); break;                                                                                       \
-    case 7: call(7); break;                                                                                       \
-    case 8: call(8); break;                                                                                       \
-    case 9: call(9); break;                                                                                       \
-    default: call(10); break;                                                                                     \
-    }
-
 template <typename TIn>
 __global__ __launch_bounds__(kBlock) void batch_normalize_kernel(BatchDev b) {
     const BatchTile tl = b.tiles[blockIdx.x];
@@ -860,9 +846,10 @@ __global__ __launch_bounds__(kBlock) void batch_normalize_kernel(BatchDev b) {
     }
 }
 
-template <typename TIn, typename TAcc>
-__global__ __launch_bounds__(kBlock) void batch_hv_kernel(BatchDev b) {
-    const BatchTile tl = b.tiles[blockIdx.x];
+template <int NS, typename TIn, typename TAcc>
+__global__ __launch_bounds__(kBlock) void batch_hv_kernel(BatchDev b, int t0) {
+    const int t = t0 + blockIdx.x;
+    const BatchTile tl = b.tiles[t];
     const NvOpDev op = b.ops[tl.prob];
     uint8_t *dig;
     const TAcc *ftab = stage_tables<TAcc>(op, dig);
@@ -876,13 +863,9 @@ __global__ __launch_bounds__(kBlock) void batch_hv_kernel(BatchDev b) {
     if (live) digits_energy(op, r, dig, tid);
     __syncthreads();
     double part = 0.0;
-    if (live) {
-#define NV_HV_ROW(n) batch_hv_row<n, TIn, TAcc>(op, x, q, y, r, dig, ftab, tid, part)
-        NV_BATCH_NS(NV_HV_ROW)
-#undef NV_HV_ROW
-    }
+    if (live) batch_hv_row<NS, TIn, TAcc>(op, x, q, y, r, dig, ftab, tid, part);
     const double bs = block_sum(part);
-    if (tid == 0) b.partials[blockIdx.x] = bs;
+    if (tid == 0) b.partials[t] = bs;
 }
 
 template <int NS, typename TAcc>
@@ -904,10 +887,11 @@ __device__ __forceinline__ void batch_step_row(const NvOpDev &op, const double *
     }
 }
 
-template <typename TAcc>
-__global__ __launch_bounds__(kBlock) void batch_step_kernel(BatchDev b) {
+template <int NS, typename TAcc>
+__global__ __launch_bounds__(kBlock) void batch_step_kernel(BatchDev b, int t0) {
     extern __shared__ __align__(16) unsigned char dig[];
-    const BatchTile tl = b.tiles[blockIdx.x];
+    const int t = t0 + blockIdx.x;
+    const BatchTile tl = b.tiles[t];
     const NvOpDev op = b.ops[tl.prob];
     const int tid = threadIdx.x;
     const int r = tl.row0 + tid;
@@ -922,18 +906,16 @@ __global__ __launch_bounds__(kBlock) void batch_step_kernel(BatchDev b) {
     double part = 0.0;
     if (r < op.D) {
         if (op.offstate) digits_energy(op, r, dig, tid);
-#define NV_STEP_ROW(n) batch_step_row<n, TAcc>(op, q, qm, res, y, a, bb, it, r, dig, tid, part)
-        NV_BATCH_NS(NV_STEP_ROW)
-#undef NV_STEP_ROW
+        batch_step_row<NS, TAcc>(op, q, qm, res, y, a, bb, it, r, dig, tid, part);
     }
     const double bs = block_sum(part);
-    if (tid == 0) b.partials[blockIdx.x] = bs;
+    if (tid == 0) b.partials[t] = bs;
 }
 
 // dst[p][it] = (sqrt of) the sum of problem p's tile partials, in tile order.
 __global__ __launch_bounds__(256) void batch_finish_kernel(BatchDev b, double *dst, int take_sqrt) {
     const int p = blockIdx.x;
-    const int t0 = b.tile0[p], t1 = b.tile0[p + 1];
+    const int t0 = b.tile0[2 * p], t1 = b.tile0[2 * p + 1];
     double s = 0.0;
     for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) s += b.partials[t];
     s = block_sum(s);
@@ -953,22 +935,53 @@ hipError_t launch_batch_normalize(const BatchDev &b, int prec, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_batch_hv(const BatchDev &b, int prec, size_t lds, hipStream_t s) {
+template <int NS>
+static void batch_hv_ns(const BatchDev &b, int prec, size_t lds, int t0, int nt, hipStream_t s) {
     if (prec == 0)
-        hipLaunchKernelGGL((batch_hv_kernel<double, double>), dim3(b.T), dim3(b.bs), lds, s, b);
+        hipLaunchKernelGGL((batch_hv_kernel<NS, double, double>), dim3(nt), dim3(b.bs), lds, s, b, t0);
     else
-        hipLaunchKernelGGL((batch_hv_kernel<float, float>), dim3(b.T), dim3(b.bs), lds, s, b);
+        hipLaunchKernelGGL((batch_hv_kernel<NS, float, float>), dim3(nt), dim3(b.bs), lds, s, b, t0);
+}
+
+template <int NS>
+static void batch_step_ns(const BatchDev &b, int prec, size_t lds, int t0, int nt, hipStream_t s) {
+    if (prec == 0)
+        hipLaunchKernelGGL((batch_step_kernel<NS, double>), dim3(nt), dim3(b.bs), lds, s, b, t0);
+    else
+        hipLaunchKernelGGL((batch_step_kernel<NS, float>), dim3(nt), dim3(b.bs), lds, s, b, t0);
+}
+
+#define NV_BATCH_DISPATCH(fn, ns, ...)                                                                            \
+    switch (ns) {                                                                                                 \
+    case 1: fn<1>(__VA_ARGS__); break;                                                                            \
+    case 2: fn<2>(__VA_ARGS__); break;                                                                            \
+    case 3: fn<3>(__VA_ARGS__); break;                                                                            \
+    case 4: fn<4>(__VA_ARGS__); break;                                                                            \
+    case 5: fn<5>(__VA_ARGS__); break;                                                                            \
+    case 6: fn<6>(__VA_ARGS__); break;                                                                            \
+    case 7: fn<7>(__VA_ARGS__); break;                                                                            \
+    case 8: fn<8>(__VA_ARGS__); break;                                                                            \
+    case 9: fn<9>(__VA_ARGS__); break;                                                                            \
+    case 10: fn<10>(__VA_ARGS__); break;                                                                          \
+    default: return hipErrorInvalidValue;                                                                         \
+    }
+
+hipError_t launch_batch_hv(const BatchDev &b, const BatchGroup *groups, int ngroups, int prec, size_t lds,
+                           hipStream_t s) {
+    for (int g = 0; g < ngroups; ++g) {
+        NV_BATCH_DISPATCH(batch_hv_ns, groups[g].ns, b, prec, lds, groups[g].t0, groups[g].nt, s)
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(batch_finish_kernel, dim3(b.P), dim3(256), 0, s, b, b.alpha, 0);
     return hipGetLastError();
 }
 
-hipError_t launch_batch_step(const BatchDev &b, int prec, size_t lds, hipStream_t s) {
-    if (prec == 0)
-        hipLaunchKernelGGL((batch_step_kernel<double>), dim3(b.T), dim3(b.bs), lds, s, b);
-    else
-        hipLaunchKernelGGL((batch_step_kernel<float>), dim3(b.T), dim3(b.bs), lds, s, b);
+hipError_t launch_batch_step(const BatchDev &b, const BatchGroup *groups, int ngroups, int prec, size_t lds,
+                             hipStream_t s) {
+    for (int g = 0; g < ngroups; ++g) {
+        NV_BATCH_DISPATCH(batch_step_ns, groups[g].ns, b, prec, lds, groups[g].t0, groups[g].nt, s)
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(batch_finish_kernel, dim3(b.P), dim3(256), 0, s, b, b.beta, 1);

@@ -99,8 +99,8 @@ struct BatchTile {
 };
 struct BatchDev {
     const NvOpDev *ops = nullptr;     // [P] operator tables
-    const BatchTile *tiles = nullptr; // [T] tiles, problem-major
-    const int32_t *tile0 = nullptr;   // [P+1] first tile of each problem
+    const BatchTile *tiles = nullptr; // [T] tiles, grouped by number of states, problem-major within
+    const int32_t *tile0 = nullptr;   // [P][2] first and one-past-last tile of each problem
     const int64_t *voff = nullptr;    // [P] offset of problem p in the state-major vectors
     const int64_t *loff = nullptr;    // [P] offset of problem p in the interleaved low-precision copy
     double *V = nullptr;              // 3 slots of Ntot doubles (slot i % 3)
@@ -113,11 +113,17 @@ struct BatchDev {
     const int64_t *iter = nullptr;    // current iteration (device)
     int32_t P = 0, T = 0, bs = 256;   // problems, tiles, rows per tile (= workgroup size)
 };
+// Tiles [t0, t0+nt) all belong to problems with ns states (one launch each).
+struct BatchGroup {
+    int ns, t0, nt;
+};
 size_t batch_hv_lds(const NvOpDev &op, int prec);
 // update_lanczos, Hv (+ alpha finish) and lanczos_step (+ beta finish) for every problem.
 hipError_t launch_batch_normalize(const BatchDev &b, int prec, hipStream_t s);
-hipError_t launch_batch_hv(const BatchDev &b, int prec, size_t lds, hipStream_t s);
-hipError_t launch_batch_step(const BatchDev &b, int prec, size_t lds, hipStream_t s);
+hipError_t launch_batch_hv(const BatchDev &b, const BatchGroup *groups, int ngroups, int prec, size_t lds,
+                           hipStream_t s);
+hipError_t launch_batch_step(const BatchDev &b, const BatchGroup *groups, int ngroups, int prec, size_t lds,
+                             hipStream_t s);
 
 // ---- adjoint (nv_adjoint.hip) ----
 int adj_blocks(int64_t N);
