@@ -23,6 +23,9 @@ struct LzState {
     double *partials = nullptr;  // per-block partial sums
     unsigned *counters = nullptr;
     const int64_t *iter = nullptr;  // current iteration (device)
+    float *Vf = nullptr;         // mixed precision with re-orthogonalisation: FP32 copy of every
+                                 // archived q_j (slot j % nslots, state-major), read by the
+                                 // overlap dots and the Gram-Schmidt updates
 };
 
 // counters[] slots

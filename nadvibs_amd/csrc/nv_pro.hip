@@ -75,6 +75,14 @@ __device__ __forceinline__ int32_t dot_count(const LzState &st, const int32_t *c
     return count_dev ? *count_dev : (int32_t)(*st.iter + count_off);
 }
 
+// TB: element type of the basis read (double: the archive itself; float: its
+// FP32 copy in mixed precision).
+template <typename TB>
+__device__ __forceinline__ const TB *basis_of(const LzState &st);
+template <> __device__ __forceinline__ const double *basis_of<double>(const LzState &st) { return st.V; }
+template <> __device__ __forceinline__ const float *basis_of<float>(const LzState &st) { return st.Vf; }
+
+template <typename TB>
 __global__ __launch_bounds__(kBlock) void pro_multidot_kernel(int64_t N, LzState st, const int64_t *jlist,
                                                               const int32_t *count_dev, int32_t count_off,
                                                               int a_cur, int b_res, double *part) {
@@ -106,11 +114,11 @@ __global__ __launch_bounds__(kBlock) void pro_multidot_kernel(int64_t N, LzState
             const int64_t c = g * kGemvG + k;
             if (c < count) {
                 const int64_t j = jlist ? jlist[c] : c + 1;
-                const double *q = st.V + (j % st.nslots) * N;
+                const TB *q = basis_of<TB>(st) + (j % st.nslots) * N;
 #pragma unroll
                 for (int e = 0; e < kGemvE; ++e) {
                     const int64_t i = base + (int64_t)e * kBlock;
-                    const double qi = i < N ? q[i] : 0.0;
+                    const double qi = i < N ? (double)q[i] : 0.0;
                     sa[k] += av[e] * qi;
                     sb[k] += bv[e] * qi;
                 }
@@ -371,12 +379,13 @@ __device__ double pro_block_sum(double v) {
 // its components along the listed vectors, subtracted in list order, then
 // beta(i) = |residual|.  Eight basis vectors are loaded before they are used
 // so that the loads overlap.
+template <typename TB>
 __global__ __launch_bounds__(kBlock) void pro_axpy_kernel(int64_t N, ProDev p, LzState st) {
     const int32_t n = *p.njl;
     if (n == 0) return;
     const int64_t it = *st.iter;
     double *w = st.V + ((it + 1) % st.nslots) * N;
-    const double *V = st.V;
+    const TB *V = basis_of<TB>(st);
     const int64_t ns = st.nslots;
     double part = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock) {
@@ -385,11 +394,11 @@ __global__ __launch_bounds__(kBlock) void pro_axpy_kernel(int64_t N, ProDev p, L
         for (; c + 8 <= n; c += 8) {
             double q[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) q[k] = V[(p.jlist[c + k] % ns) * N + i];
+            for (int k = 0; k < 8; ++k) q[k] = (double)V[(p.jlist[c + k] % ns) * N + i];
 #pragma unroll
             for (int k = 0; k < 8; ++k) v = v - p.dots_b[c + k] * q[k];
         }
-        for (; c < n; ++c) v = v - p.dots_b[c] * V[(p.jlist[c] % ns) * N + i];
+        for (; c < n; ++c) v = v - p.dots_b[c] * (double)V[(p.jlist[c] % ns) * N + i];
         w[i] = v;
         part += v * v;
     }
@@ -483,8 +492,12 @@ hipError_t pro_reset(ProState &ps, const nv_lanczos_opts &o, hipStream_t s) {
 static hipError_t multidot(const ProState &ps, int64_t N, const LzState &st, const int64_t *jlist,
                            const int32_t *count_dev, int32_t count_off, int a_cur, int b_res, hipStream_t s) {
     const ProDev &p = ps.d;
-    hipLaunchKernelGGL(pro_multidot_kernel, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev, count_off,
-                       a_cur, b_res, p.dpart);
+    if (st.Vf)
+        hipLaunchKernelGGL(pro_multidot_kernel<float>, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev,
+                           count_off, a_cur, b_res, p.dpart);
+    else
+        hipLaunchKernelGGL(pro_multidot_kernel<double>, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev,
+                           count_off, a_cur, b_res, p.dpart);
     hipLaunchKernelGGL(pro_reduce_kernel, dim3(256), dim3(kBlock), 0, s, N, st, count_dev, count_off, p.dpart,
                        p.dots_a, p.dots_b);
     return hipGetLastError();
@@ -524,7 +537,10 @@ hipError_t pro_partial_orthog(ProState &ps, const NvOpDev &dev, const LzState &s
     if (e != hipSuccess) return e;
     const int64_t N = dev.N;
     int g = (int)std::min<int64_t>((N + kBlock - 1) / kBlock, 4096);
-    hipLaunchKernelGGL(pro_axpy_kernel, dim3(g), dim3(kBlock), 0, s, N, p, st);
+    if (st.Vf)
+        hipLaunchKernelGGL(pro_axpy_kernel<float>, dim3(g), dim3(kBlock), 0, s, N, p, st);
+    else
+        hipLaunchKernelGGL(pro_axpy_kernel<double>, dim3(g), dim3(kBlock), 0, s, N, p, st);
     hipLaunchKernelGGL(pro_beta_kernel, dim3(1), dim3(kBlock), 0, s, p, st, g);
     return hipGetLastError();
 }
