@@ -419,7 +419,11 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
             NV_HIP(c, alloc(lo_bytes, &st.vlo));
             NV_HIP(c, hipMemset(st.vlo, 0, lo_bytes));
             // re-orthogonalisation reads the FP32 copy of the archive (half the bytes)
-            if (o->reorthog > 0) NV_HIP(c, alloc((size_t)st.nslots * N * sizeof(float), (void **)&st.Vf));
+            if (o->reorthog > 0) {
+                st.ldf = N + (N & 1);
+                NV_HIP(c, alloc((size_t)st.nslots * st.ldf * sizeof(float), (void **)&st.Vf));
+                NV_HIP(c, hipMemset(st.Vf, 0, (size_t)st.nslots * st.ldf * sizeof(float)));
+            }
         }
         NV_HIP(c, alloc((size_t)N * (o->prec == NV_PREC_F64 ? 8 : 4), &st.y));
         NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.alpha));

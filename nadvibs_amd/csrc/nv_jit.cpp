@@ -42,6 +42,9 @@ std::string jit_source(const Operator &op, int prec) {
     std::ostringstream s;
     // hiprtc provides the HIP device runtime itself (no includes)
     s << "typedef " << T << " T;\n";
+    // fused multiply-add in the accumulation type (the double builtin would
+    // promote FP32 operands and convert back around every step)
+    s << (f32 ? "#define NV_FMA __builtin_fmaf\n" : "#define NV_FMA __builtin_fma\n");
     s << "#define NV_D " << D << "LL\n#define NV_N " << op.N << "LL\n";
     // factor tables of the codes used by the steps
     // code -> (mode, offset into TAB)
@@ -141,21 +144,21 @@ __device__ __forceinline__ double nv_block_sum(double v) {
     // the loads of one step: w_q += h * x[q][r + stride]
     auto step_loads = [&](std::ostringstream &o) {
         if (!f32) {
-            for (int q = 0; q < ns; ++q) o << " w" << q << " = __builtin_fma(h, nv_ld(rs" << q << ", o), w" << q << ");";
+            for (int q = 0; q < ns; ++q) o << " w" << q << " = NV_FMA(h, nv_ld(rs" << q << ", o), w" << q << ");";
             return;
         }
         if (nsp == 1) {
-            o << " w0 = __builtin_fma(h, nv_ld1(rx, o), w0);";
+            o << " w0 = NV_FMA(h, nv_ld1(rx, o), w0);";
         } else if (nsp == 2) {
             o << " { const nv_f2 v = nv_ld2(rx, o);";
-            for (int q = 0; q < ns; ++q) o << " w" << q << " = __builtin_fma(h, v[" << q << "], w" << q << ");";
+            for (int q = 0; q < ns; ++q) o << " w" << q << " = NV_FMA(h, v[" << q << "], w" << q << ");";
             o << " }";
         } else {
             for (int part = 0; part < nsp / 4; ++part) {
                 if (part * 4 >= ns) break;
                 o << " { const nv_f4 v = nv_ld4(rx, o + " << part * 16 << "u);";
                 for (int q = part * 4; q < ns && q < part * 4 + 4; ++q)
-                    o << " w" << q << " = __builtin_fma(h, v[" << q - part * 4 << "], w" << q << ");";
+                    o << " w" << q << " = NV_FMA(h, v[" << q - part * 4 << "], w" << q << ");";
                 o << " }";
             }
         }
@@ -176,7 +179,7 @@ __device__ __forceinline__ double nv_block_sum(double v) {
         for (int a = 0; a < ns; ++a)
             for (int c = 0; c < ns; ++c)
                 if (C[a * ns + c] != 0.0)
-                    b << "        acc" << a << " = __builtin_fma((T)" << lit(C[a * ns + c], f32) << ", w" << c
+                    b << "        acc" << a << " = NV_FMA((T)" << lit(C[a * ns + c], f32) << ", w" << c
                       << ", acc" << a << ");\n";
         b << "    }\n";
     }

@@ -331,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void normalize_kernel(int64_t N, int64_t D,
             const int64_t s = i / D;
             lo[(i - s * D) * nsp + s] = from_d<TIn>(v);
         }
-        if (st.Vf) st.Vf[slot_of(it, st.nslots) * N + i] = (float)v;
+        if (st.Vf) st.Vf[slot_of(it, st.nslots) * st.ldf + i] = (float)v;
     }
 }
 

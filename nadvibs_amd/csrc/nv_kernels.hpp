@@ -24,8 +24,9 @@ struct LzState {
     unsigned *counters = nullptr;
     const int64_t *iter = nullptr;  // current iteration (device)
     float *Vf = nullptr;         // mixed precision with re-orthogonalisation: FP32 copy of every
-                                 // archived q_j (slot j % nslots, state-major), read by the
-                                 // overlap dots and the Gram-Schmidt updates
+                                 // archived q_j (slot j % nslots, state-major, slot stride ldf),
+                                 // read by the overlap dots and the Gram-Schmidt updates
+    int64_t ldf = 0;             // N rounded up to even (pairs of FP32 elements load as one)
 };
 
 // counters[] slots

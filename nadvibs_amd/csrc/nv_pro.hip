@@ -75,12 +75,89 @@ __device__ __forceinline__ int32_t dot_count(const LzState &st, const int32_t *c
     return count_dev ? *count_dev : (int32_t)(*st.iter + count_off);
 }
 
-// TB: element type of the basis read (double: the archive itself; float: its
-// FP32 copy in mixed precision).
+// TB: element type of the basis read (double: the archive itself, slot
+// stride N; float: its FP32 copy in mixed precision, slot stride ldf).
 template <typename TB>
 __device__ __forceinline__ const TB *basis_of(const LzState &st);
 template <> __device__ __forceinline__ const double *basis_of<double>(const LzState &st) { return st.V; }
 template <> __device__ __forceinline__ const float *basis_of<float>(const LzState &st) { return st.Vf; }
+template <typename TB>
+__device__ __forceinline__ int64_t stride_of(const LzState &st, int64_t N);
+template <> __device__ __forceinline__ int64_t stride_of<double>(const LzState &, int64_t N) { return N; }
+template <> __device__ __forceinline__ int64_t stride_of<float>(const LzState &st, int64_t) { return st.ldf; }
+
+// The same dots against the FP32 copy: each lane loads element pairs (8-byte
+// loads, as the FP64 kernel), over the same chunks.
+__global__ __launch_bounds__(kBlock) void pro_multidot_f32_kernel(int64_t N, LzState st, const int64_t *jlist,
+                                                                  const int32_t *count_dev, int32_t count_off,
+                                                                  int a_cur, int b_res, double *part) {
+    constexpr int kPairs = kGemvE / 2;
+    __shared__ double red[kBlock / 64][2 * kGemvG];
+    const int64_t it = *st.iter;
+    const int32_t count = dot_count(st, count_dev, count_off);
+    if (count <= 0) return;
+    const double *a = a_cur ? st.V + (it % st.nslots) * N : nullptr;
+    const double *b = b_res ? st.V + ((it + 1) % st.nslots) * N : nullptr;
+    const int64_t nchunks = (N + kGemvCh - 1) / kGemvCh;
+    const int64_t ngroups = (count + kGemvG - 1) / kGemvG;
+    const int64_t items = ngroups * nchunks;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int64_t item = blockIdx.x; item < items; item += gridDim.x) {
+        const int64_t g = item / nchunks, ch = item - g * nchunks;
+        const int64_t pbase = ch * (kGemvCh / 2) + threadIdx.x;   // pair index
+        double a0[kPairs], a1[kPairs], b0[kPairs], b1[kPairs];
+#pragma unroll
+        for (int e = 0; e < kPairs; ++e) {
+            const int64_t i = 2 * (pbase + (int64_t)e * kBlock);
+            a0[e] = (a && i < N) ? a[i] : 0.0;
+            a1[e] = (a && i + 1 < N) ? a[i + 1] : 0.0;
+            b0[e] = (b && i < N) ? b[i] : 0.0;
+            b1[e] = (b && i + 1 < N) ? b[i + 1] : 0.0;
+        }
+        double sa[kGemvG], sb[kGemvG];
+#pragma unroll
+        for (int k = 0; k < kGemvG; ++k) {
+            sa[k] = 0.0;
+            sb[k] = 0.0;
+            const int64_t c = g * kGemvG + k;
+            if (c < count) {
+                const int64_t j = jlist ? jlist[c] : c + 1;
+                const float2 *q = reinterpret_cast<const float2 *>(st.Vf + (j % st.nslots) * st.ldf);
+#pragma unroll
+                for (int e = 0; e < kPairs; ++e) {
+                    const int64_t pp = pbase + (int64_t)e * kBlock;
+                    // pairs past N read the zero padding of the slot (ldf is even)
+                    const float2 v = 2 * pp < N ? q[pp] : make_float2(0.0f, 0.0f);
+                    sa[k] += a0[e] * (double)v.x + a1[e] * (double)v.y;
+                    sb[k] += b0[e] * (double)v.x + b1[e] * (double)v.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kGemvG; ++k) {
+            for (int off = 32; off > 0; off >>= 1) {
+                sa[k] += __shfl_xor(sa[k], off, 64);
+                sb[k] += __shfl_xor(sb[k], off, 64);
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < kGemvG; ++k) {
+                red[wid][k] = sa[k];
+                red[wid][kGemvG + k] = sb[k];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * kGemvG) {
+            const int k = threadIdx.x % kGemvG, which = threadIdx.x / kGemvG;
+            const int64_t c = g * kGemvG + k;
+            double t = 0.0;
+            for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
+            if (c < count) part[(c * nchunks + ch) * 2 + which] = t;
+        }
+    }
+}
 
 template <typename TB>
 __global__ __launch_bounds__(kBlock) void pro_multidot_kernel(int64_t N, LzState st, const int64_t *jlist,
@@ -114,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void pro_multidot_kernel(int64_t N, LzState
             const int64_t c = g * kGemvG + k;
             if (c < count) {
                 const int64_t j = jlist ? jlist[c] : c + 1;
-                const TB *q = basis_of<TB>(st) + (j % st.nslots) * N;
+                const TB *q = basis_of<TB>(st) + (j % st.nslots) * stride_of<TB>(st, N);
 #pragma unroll
                 for (int e = 0; e < kGemvE; ++e) {
                     const int64_t i = base + (int64_t)e * kBlock;
@@ -386,6 +463,7 @@ __global__ __launch_bounds__(kBlock) void pro_axpy_kernel(int64_t N, ProDev p, L
     const int64_t it = *st.iter;
     double *w = st.V + ((it + 1) % st.nslots) * N;
     const TB *V = basis_of<TB>(st);
+    const int64_t ld = stride_of<TB>(st, N);
     const int64_t ns = st.nslots;
     double part = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock) {
@@ -394,11 +472,11 @@ __global__ __launch_bounds__(kBlock) void pro_axpy_kernel(int64_t N, ProDev p, L
         for (; c + 8 <= n; c += 8) {
             double q[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) q[k] = (double)V[(p.jlist[c + k] % ns) * N + i];
+            for (int k = 0; k < 8; ++k) q[k] = (double)V[(p.jlist[c + k] % ns) * ld + i];
 #pragma unroll
             for (int k = 0; k < 8; ++k) v = v - p.dots_b[c + k] * q[k];
         }
-        for (; c < n; ++c) v = v - p.dots_b[c] * (double)V[(p.jlist[c] % ns) * N + i];
+        for (; c < n; ++c) v = v - p.dots_b[c] * (double)V[(p.jlist[c] % ns) * ld + i];
         w[i] = v;
         part += v * v;
     }
@@ -493,7 +571,7 @@ static hipError_t multidot(const ProState &ps, int64_t N, const LzState &st, con
                            const int32_t *count_dev, int32_t count_off, int a_cur, int b_res, hipStream_t s) {
     const ProDev &p = ps.d;
     if (st.Vf)
-        hipLaunchKernelGGL(pro_multidot_kernel<float>, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev,
+        hipLaunchKernelGGL(pro_multidot_f32_kernel, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev,
                            count_off, a_cur, b_res, p.dpart);
     else
         hipLaunchKernelGGL(pro_multidot_kernel<double>, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev,
