@@ -23,6 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Lanczos iter/s + H*v HBM GB/s (%peak); mixed vs FP64 at <=1e-3 cm^-1 error"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
 AU2WAVE = 219474.6313705
 
 
@@ -163,6 +164,48 @@ def batched_cfg5(sync, cpu_seconds):
                                                        f"{dt:.1f} s"}}
 
 
+def adjoint_cfg4(sync):
+    """cfg4: the cfg2 operator, 500 iterations; loss |G*sticks - target|_2
+    with a Gaussian of sigma 15 cm^-1 on 4096 points over 0-6000 cm^-1 above
+    the lowest root; target = the spectrum of the couplings scaled by
+    1 + N(0, 0.05).  Gradient with respect to every nadvibs.in coefficient and
+    frequency through the Krylov-projected adjoint, in FP64 and with FP32 H*v."""
+    import torch
+    from nadvibs_amd.adjoint import spectrum_of_parameters
+    from nadvibs_amd.synth import workload
+    p = workload("cfg2")
+    niter = 500
+    grid = torch.linspace(0.0, 6000.0, 4096, dtype=torch.float64)
+    rng = np.random.default_rng(1004)
+    w0 = torch.tensor(p.aomega, dtype=torch.float64)
+    c0 = torch.tensor(p.concoef, dtype=torch.float64)
+    pt0 = torch.tensor(p.potterms, dtype=torch.float64)
+    with torch.no_grad():
+        target = spectrum_of_parameters(p, w0, c0, pt0 * (1 + 0.05 * torch.from_numpy(rng.standard_normal(pt0.shape))),
+                                        niter, grid, 15.0, "lowest")
+    out, grads = {}, {}
+    for prec in ("f64", "f32"):
+        w, c, pt = (t.clone().requires_grad_(True) for t in (w0, c0, pt0))
+        sync()
+        t0 = time.perf_counter()
+        env = spectrum_of_parameters(p, w, c, pt, niter, grid, 15.0, "lowest", prec=prec)
+        loss = torch.linalg.vector_norm(env - target)
+        t1 = time.perf_counter()
+        loss.backward()
+        t2 = time.perf_counter()
+        g = torch.cat([w.grad.reshape(-1), c.grad.reshape(-1), pt.grad.reshape(-1)]).numpy()
+        grads[prec] = g
+        out[prec] = {"forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3), "loss": float(loss)}
+        print(f"# adjoint cfg4 {prec}: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s, loss {float(loss):.6g}",
+              file=sys.stderr, flush=True)
+    d = grads["f32"] - grads["f64"]
+    out.update({"workload": "cfg4: cfg2 operator, 500 iterations, sigma 15 cm^-1, 4096 points",
+                "parameters": int(grads["f64"].size), "nonzero_gradients": int(np.count_nonzero(grads["f64"])),
+                "max_rel_diff_f32_vs_f64": float(f"{np.max(np.abs(d)) / np.max(np.abs(grads['f64'])):.3e}"),
+                "gradient": "Krylov-projected (nv_spectrum_adjoint + nv_krylov_gradient), table-driven H*v"})
+    return out
+
+
 def load_traffic(path):
     """HBM bytes per H*v launch measured with rocprofv3 counters, if a
     summary with a 'traffic_bytes_per_launch:' line exists."""
@@ -211,6 +254,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-problem batched workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the spectrum-gradient workload")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only the calibration copy and H*v launches, for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -253,6 +297,7 @@ def main():
             eng.update_lanczos(1)
             burst[p] = eng.profile_hv(200)
         print(f"# H*v back-to-back: {burst}", file=sys.stderr, flush=True)
+        hv_flops = eng.hv_flops()
 
     iters = args.steps * args.niter
     value = iters / dt32
@@ -274,6 +319,9 @@ def main():
                      "bytes_per_launch": hv_bytes, "launches": 200,
                      "in_job_avg_launch_us": round(hv32_ms / max(hv32_n, 1) * 1e3, 2)},
         "hv_gbps": round(achieved, 1),
+        "hv_compute": {"flops_per_launch": hv_flops, "achieved_tflops": round(hv_flops / hv_avg_s / 1e12, 2),
+                       "peak_tflops_fp32": FP32_PEAK_TFLOPS,
+                       "frac": round(hv_flops / hv_avg_s / 1e12 / FP32_PEAK_TFLOPS, 4)},
         "pro": {"events_per_job": pro32[0], "vectors_touched_per_job": pro32[1]},
     }
     if res64 is not None:
@@ -291,6 +339,8 @@ def main():
         rec["error_target_met"] = bool(dE <= 1e-3)
     if not args.no_batch:
         rec["batched_cfg5"] = batched_cfg5(sync, 0.0 if args.no_cpu else args.cpu_seconds)
+    if not args.no_adjoint:
+        rec["adjoint_cfg4"] = adjoint_cfg4(sync)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
     print(json.dumps(rec), flush=True)

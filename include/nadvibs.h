@@ -312,6 +312,9 @@ int32_t nv_profile_enable(nv_ctx *ctx, int32_t on);
 /* Milliseconds spent in the timed H*v launches and their number since the
  * previous call; synchronises. */
 int32_t nv_profile_read(nv_ctx *ctx, double *hv_ms, int64_t *hv_launches);
+/* Floating-point operations of one H*v: twice the multiply-adds of its
+ * nonzero structure (every row, step and coupled state pair). */
+int32_t nv_hv_flops(const nv_ctx *ctx, int64_t *flops);
 /* Average duration in microseconds of `launches` back-to-back launches of
  * the iteration H*v kernel alone (after nv_update_lanczos(1)), timed with one
  * pair of HIP events on the context's stream.  Overwrites alpha(1). */

@@ -76,6 +76,7 @@ SIGNATURES = [
     ("nv_krylov_gradient", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, _dp, _dp]),
     ("nv_select_hv_kernel", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     ("nv_profile_hv", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp]),
+    ("nv_hv_flops", ctypes.c_int32, [ctypes.c_void_p, _i64p]),
     ("nv_batch_create", ctypes.c_int32, [ctypes.POINTER(NvProblem), ctypes.c_int64, ctypes.c_int32,
                                          ctypes.POINTER(ctypes.c_void_p)]),
     ("nv_batch_destroy", ctypes.c_int32, [ctypes.c_void_p]),

@@ -520,6 +520,12 @@ int32_t nv_profile_enable(nv_ctx *c, int32_t on) {
     return NV_OK;
 }
 
+int32_t nv_hv_flops(const nv_ctx *c, int64_t *flops) {
+    if (!c || !flops) return NV_ERR_ARG;
+    *flops = 2 * c->op.nnz_fma;
+    return NV_OK;
+}
+
 int32_t nv_profile_hv(nv_ctx *c, int64_t launches, double *avg_us) {
     if (!c || !avg_us || launches < 1) return NV_ERR_ARG;
     if (!c->lz_ready || c->next_update < 2) return fail(c, NV_ERR_STATE, "update_lanczos(1) must come first");

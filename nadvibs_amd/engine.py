@@ -268,6 +268,12 @@ class Engine:
     def profile(self, on=True):
         self._check(self.lib.nv_profile_enable(self.h, 1 if on else 0))
 
+    def hv_flops(self):
+        """Floating-point operations of one H*v (2 x nonzero multiply-adds)."""
+        f = ctypes.c_int64()
+        self._check(self.lib.nv_hv_flops(self.h, ctypes.byref(f)))
+        return f.value
+
     def profile_hv(self, launches=200):
         """Average microseconds of back-to-back iteration H*v launches (after update_lanczos(1))."""
         us = ctypes.c_double()
