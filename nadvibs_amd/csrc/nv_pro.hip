@@ -62,21 +62,25 @@ __global__ __launch_bounds__(kBlock) void pro_recur_kernel(ProDev p, LzState st)
 
 // Dots of a and b (either may be null) with the Krylov vectors q_j listed in
 // jlist (or j = c+1 when jlist is null), c < count: a block GEMV over the
-// HBM-resident basis.  Work items are (group of kGemvG vectors) x (chunk of
-// kGemvCh elements); a persistent grid walks them, so the launch shape does
-// not depend on the device-side count.  Each item writes one partial per dot;
-// pro_reduce_kernel sums the chunks of a dot in chunk order.
-constexpr int kGemvG = 8;
-constexpr int kGemvE = 8;
-constexpr int kGemvCh = kBlock * kGemvE;
-
-// count: *count_dev, or iter + count_off when count_dev is null.
-__device__ __forceinline__ int32_t dot_count(const LzState &st, const int32_t *count_dev, int32_t count_off) {
-    return count_dev ? *count_dev : (int32_t)(*st.iter + count_off);
-}
+// HBM-resident basis.  Work items are (group of G vectors) x (segment of the
+// vectors).  The number of segments follows the device-side count, so that
+// about kDotItems items exist for any count, each segment holding at least
+// kDotMinSeg and at most kBlock * kDotMaxPasses load units.  A lane keeps its
+// sums over the whole segment in registers and the block reduces them once
+// per item; pro_reduce_kernel sums the segments of a dot in segment order.
+// The a and b loads are shared by the G vectors of a group.
+constexpr int kGemvCh = 2048;        // elements; dpart holds ceil(N / kGemvCh) segments per dot
+constexpr int kDotItems = 2048;
+constexpr int kDotMinSeg = kGemvCh;  // elements
+constexpr int kDotMaxPasses = 64;
 
 // TB: element type of the basis read (double: the archive itself, slot
-// stride N; float: its FP32 copy in mixed precision, slot stride ldf).
+// stride N; float: its FP32 copy in mixed precision, slot stride ldf, read in
+// element pairs: ldf is even and the pad element is zero).
+template <typename TB> struct DotBasis;
+template <> struct DotBasis<float> { static constexpr int W = 2, G = 16; };
+template <> struct DotBasis<double> { static constexpr int W = 1, G = 8; };
+
 template <typename TB>
 __device__ __forceinline__ const TB *basis_of(const LzState &st);
 template <> __device__ __forceinline__ const double *basis_of<double>(const LzState &st) { return st.V; }
@@ -86,76 +90,101 @@ __device__ __forceinline__ int64_t stride_of(const LzState &st, int64_t N);
 template <> __device__ __forceinline__ int64_t stride_of<double>(const LzState &, int64_t N) { return N; }
 template <> __device__ __forceinline__ int64_t stride_of<float>(const LzState &st, int64_t) { return st.ldf; }
 
-// The same dots against the FP32 copy: each lane loads element pairs (8-byte
-// loads, as the FP64 kernel), over the same chunks.
-__global__ __launch_bounds__(kBlock) void pro_multidot_f32_kernel(int64_t N, LzState st, const int64_t *jlist,
-                                                                  const int32_t *count_dev, int32_t count_off,
-                                                                  int a_cur, int b_res, double *part) {
-    constexpr int kPairs = kGemvE / 2;
-    __shared__ double red[kBlock / 64][2 * kGemvG];
-    const int64_t it = *st.iter;
-    const int32_t count = dot_count(st, count_dev, count_off);
-    if (count <= 0) return;
-    const double *a = a_cur ? st.V + (it % st.nslots) * N : nullptr;
-    const double *b = b_res ? st.V + ((it + 1) % st.nslots) * N : nullptr;
-    const int64_t nchunks = (N + kGemvCh - 1) / kGemvCh;
-    const int64_t ngroups = (count + kGemvG - 1) / kGemvG;
-    const int64_t items = ngroups * nchunks;
+// count: *count_dev, or iter + count_off when count_dev is null.
+__device__ __forceinline__ int32_t dot_count(const LzState &st, const int32_t *count_dev, int32_t count_off) {
+    return count_dev ? *count_dev : (int32_t)(*st.iter + count_off);
+}
+
+// Segments of a multidot over `count` vectors; seg_units: load units (W
+// elements) per segment, a multiple of kBlock.  At most ceil(N / kGemvCh).
+template <typename TB>
+__device__ __forceinline__ int64_t dot_segments(int64_t N, int32_t count, int64_t &seg_units) {
+    constexpr int W = DotBasis<TB>::W, G = DotBasis<TB>::G;
+    const int64_t units = (N + W - 1) / W;
+    const int64_t ngroups = (count + G - 1) / G;
+    const int64_t lo = (units + (int64_t)kBlock * kDotMaxPasses - 1) / ((int64_t)kBlock * kDotMaxPasses);
+    const int64_t hi = (units + kDotMinSeg / W - 1) / (kDotMinSeg / W);
+    int64_t nseg = (kDotItems + ngroups - 1) / ngroups;
+    nseg = nseg < lo ? lo : (nseg > hi ? hi : nseg);
+    int64_t per = (units + nseg - 1) / nseg;
+    per = (per + kBlock - 1) / kBlock * kBlock;
+    seg_units = per;
+    return (units + per - 1) / per;
+}
+
+// One work item: the dots of a and b with the nk <= G vectors c0 .. c0+nk-1
+// over segment sg.  Slots k >= nk repeat the last vector (their loads hit the
+// cache and their sums are dropped): the loads stay unconditional, which
+// keeps them batched.
+template <typename TB, int G, int GMAX>
+__device__ __forceinline__ void dot_item(int64_t N, const LzState &st, const double *a, const double *b,
+                                         const int64_t *jlist, int64_t c0, int nk, int64_t sg, int64_t seg_units,
+                                         int64_t nseg, double (*red)[2 * GMAX], double *part) {
+    constexpr int W = DotBasis<TB>::W;
+    const TB *basis = basis_of<TB>(st);
+    const int64_t ld = stride_of<TB>(st, N);
+    const int64_t units = (N + W - 1) / W;
+    const TB *q[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const int64_t c = c0 + (k < nk ? k : nk - 1);
+        q[k] = basis + ((jlist ? jlist[c] : c + 1) % st.nslots) * ld;
+    }
+    double sa[G], sb[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        sa[k] = 0.0;
+        sb[k] = 0.0;
+    }
+    const int64_t u_end = (sg + 1) * seg_units < units ? (sg + 1) * seg_units : units;
+    for (int64_t u = sg * seg_units + threadIdx.x; u < u_end; u += kBlock) {
+        if constexpr (W == 2) {
+            const int64_t i = 2 * u;
+            const bool two = i + 1 < N;
+            const double a0 = a ? a[i] : 0.0, a1 = (a && two) ? a[i + 1] : 0.0;
+            const double b0 = b ? b[i] : 0.0, b1 = (b && two) ? b[i + 1] : 0.0;
+            float2 v[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) v[k] = reinterpret_cast<const float2 *>(q[k])[u];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                sa[k] += a0 * (double)v[k].x + a1 * (double)v[k].y;
+                sb[k] += b0 * (double)v[k].x + b1 * (double)v[k].y;
+            }
+        } else {
+            const double av = a ? a[u] : 0.0, bv = b ? b[u] : 0.0;
+            double v[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) v[k] = (double)q[k][u];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                sa[k] += av * v[k];
+                sb[k] += bv * v[k];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            sa[k] += __shfl_xor(sa[k], off, 64);
+            sb[k] += __shfl_xor(sb[k], off, 64);
+        }
+    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int64_t item = blockIdx.x; item < items; item += gridDim.x) {
-        const int64_t g = item / nchunks, ch = item - g * nchunks;
-        const int64_t pbase = ch * (kGemvCh / 2) + threadIdx.x;   // pair index
-        double a0[kPairs], a1[kPairs], b0[kPairs], b1[kPairs];
+    __syncthreads();
+    if (lane == 0) {
 #pragma unroll
-        for (int e = 0; e < kPairs; ++e) {
-            const int64_t i = 2 * (pbase + (int64_t)e * kBlock);
-            a0[e] = (a && i < N) ? a[i] : 0.0;
-            a1[e] = (a && i + 1 < N) ? a[i + 1] : 0.0;
-            b0[e] = (b && i < N) ? b[i] : 0.0;
-            b1[e] = (b && i + 1 < N) ? b[i + 1] : 0.0;
+        for (int k = 0; k < G; ++k) {
+            red[wid][k] = sa[k];
+            red[wid][G + k] = sb[k];
         }
-        double sa[kGemvG], sb[kGemvG];
-#pragma unroll
-        for (int k = 0; k < kGemvG; ++k) {
-            sa[k] = 0.0;
-            sb[k] = 0.0;
-            const int64_t c = g * kGemvG + k;
-            if (c < count) {
-                const int64_t j = jlist ? jlist[c] : c + 1;
-                const float2 *q = reinterpret_cast<const float2 *>(st.Vf + (j % st.nslots) * st.ldf);
-#pragma unroll
-                for (int e = 0; e < kPairs; ++e) {
-                    const int64_t pp = pbase + (int64_t)e * kBlock;
-                    // pairs past N read the zero padding of the slot (ldf is even)
-                    const float2 v = 2 * pp < N ? q[pp] : make_float2(0.0f, 0.0f);
-                    sa[k] += a0[e] * (double)v.x + a1[e] * (double)v.y;
-                    sb[k] += b0[e] * (double)v.x + b1[e] * (double)v.y;
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kGemvG; ++k) {
-            for (int off = 32; off > 0; off >>= 1) {
-                sa[k] += __shfl_xor(sa[k], off, 64);
-                sb[k] += __shfl_xor(sb[k], off, 64);
-            }
-        }
-        __syncthreads();
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < kGemvG; ++k) {
-                red[wid][k] = sa[k];
-                red[wid][kGemvG + k] = sb[k];
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < 2 * kGemvG) {
-            const int k = threadIdx.x % kGemvG, which = threadIdx.x / kGemvG;
-            const int64_t c = g * kGemvG + k;
-            double t = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
-            if (c < count) part[(c * nchunks + ch) * 2 + which] = t;
-        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * G) {
+        const int k = threadIdx.x % G, which = threadIdx.x / G;
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
+        if (k < nk) part[((c0 + k) * nseg + sg) * 2 + which] = t;
     }
 }
 
@@ -163,84 +192,49 @@ template <typename TB>
 __global__ __launch_bounds__(kBlock) void pro_multidot_kernel(int64_t N, LzState st, const int64_t *jlist,
                                                               const int32_t *count_dev, int32_t count_off,
                                                               int a_cur, int b_res, double *part) {
-    __shared__ double red[kBlock / 64][2 * kGemvG];
+    constexpr int G = DotBasis<TB>::G;
+    __shared__ double red[kBlock / 64][2 * G];
     const int64_t it = *st.iter;
     const int32_t count = dot_count(st, count_dev, count_off);
     if (count <= 0) return;
     const double *a = a_cur ? st.V + (it % st.nslots) * N : nullptr;
     const double *b = b_res ? st.V + ((it + 1) % st.nslots) * N : nullptr;
-    const int64_t nchunks = (N + kGemvCh - 1) / kGemvCh;
-    const int64_t ngroups = (count + kGemvG - 1) / kGemvG;
-    const int64_t items = ngroups * nchunks;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int64_t seg_units;
+    const int64_t nseg = dot_segments<TB>(N, count, seg_units);
+    const int64_t ngroups = (count + G - 1) / G;
+    const int64_t items = ngroups * nseg;
     for (int64_t item = blockIdx.x; item < items; item += gridDim.x) {
-        const int64_t g = item / nchunks, ch = item - g * nchunks;
-        const int64_t base = ch * kGemvCh + threadIdx.x;
-        double av[kGemvE], bv[kGemvE];
-#pragma unroll
-        for (int e = 0; e < kGemvE; ++e) {
-            const int64_t i = base + (int64_t)e * kBlock;
-            av[e] = (a && i < N) ? a[i] : 0.0;
-            bv[e] = (b && i < N) ? b[i] : 0.0;
-        }
-        double sa[kGemvG], sb[kGemvG];
-#pragma unroll
-        for (int k = 0; k < kGemvG; ++k) {
-            sa[k] = 0.0;
-            sb[k] = 0.0;
-            const int64_t c = g * kGemvG + k;
-            if (c < count) {
-                const int64_t j = jlist ? jlist[c] : c + 1;
-                const TB *q = basis_of<TB>(st) + (j % st.nslots) * stride_of<TB>(st, N);
-#pragma unroll
-                for (int e = 0; e < kGemvE; ++e) {
-                    const int64_t i = base + (int64_t)e * kBlock;
-                    const double qi = i < N ? (double)q[i] : 0.0;
-                    sa[k] += av[e] * qi;
-                    sb[k] += bv[e] * qi;
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kGemvG; ++k) {
-            for (int off = 32; off > 0; off >>= 1) {
-                sa[k] += __shfl_xor(sa[k], off, 64);
-                sb[k] += __shfl_xor(sb[k], off, 64);
-            }
-        }
-        __syncthreads();
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < kGemvG; ++k) {
-                red[wid][k] = sa[k];
-                red[wid][kGemvG + k] = sb[k];
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < 2 * kGemvG) {
-            const int k = threadIdx.x % kGemvG, which = threadIdx.x / kGemvG;
-            const int64_t c = g * kGemvG + k;
-            double t = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) t += red[w][threadIdx.x];
-            if (c < count) part[(c * nchunks + ch) * 2 + which] = t;
-        }
+        const int64_t g = item / nseg, sg = item - g * nseg;
+        const int nk = (int)((count - g * G) < G ? (count - g * G) : G);
+        // the last group runs at the narrowest width that holds it
+        if (nk > G / 2)
+            dot_item<TB, G, G>(N, st, a, b, jlist, g * G, nk, sg, seg_units, nseg, red, part);
+        else if (G > 8 && nk > 4)
+            dot_item<TB, (G > 8 ? 8 : G), G>(N, st, a, b, jlist, g * G, nk, sg, seg_units, nseg, red, part);
+        else if (nk > 2)
+            dot_item<TB, 4, G>(N, st, a, b, jlist, g * G, nk, sg, seg_units, nseg, red, part);
+        else
+            dot_item<TB, 2, G>(N, st, a, b, jlist, g * G, nk, sg, seg_units, nseg, red, part);
     }
 }
 
-// One wave per dot: sums its chunk partials in chunk order.
+// One wave per dot: sums its segment partials in segment order.
+template <typename TB>
 __global__ __launch_bounds__(kBlock) void pro_reduce_kernel(int64_t N, LzState st, const int32_t *count_dev,
                                                             int32_t count_off, const double *part, double *out_a,
                                                             double *out_b) {
     const int32_t count = dot_count(st, count_dev, count_off);
-    const int64_t nchunks = (N + kGemvCh - 1) / kGemvCh;
+    if (count <= 0) return;
+    int64_t seg_units;
+    const int64_t nseg = dot_segments<TB>(N, count, seg_units);
     const int lane = threadIdx.x & 63;
     const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t c = w0; c < count; c += nw) {
         double sa = 0.0, sb = 0.0;
-        for (int64_t k = lane; k < nchunks; k += 64) {
-            sa += part[(c * nchunks + k) * 2 + 0];
-            sb += part[(c * nchunks + k) * 2 + 1];
+        for (int64_t k = lane; k < nseg; k += 64) {
+            sa += part[(c * nseg + k) * 2 + 0];
+            sb += part[(c * nseg + k) * 2 + 1];
         }
         for (int off = 32; off > 0; off >>= 1) {
             sa += __shfl_xor(sa, off, 64);
@@ -570,14 +564,17 @@ hipError_t pro_reset(ProState &ps, const nv_lanczos_opts &o, hipStream_t s) {
 static hipError_t multidot(const ProState &ps, int64_t N, const LzState &st, const int64_t *jlist,
                            const int32_t *count_dev, int32_t count_off, int a_cur, int b_res, hipStream_t s) {
     const ProDev &p = ps.d;
-    if (st.Vf)
-        hipLaunchKernelGGL(pro_multidot_f32_kernel, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev,
+    if (st.Vf) {
+        hipLaunchKernelGGL(pro_multidot_kernel<float>, dim3(kDotItems), dim3(kBlock), 0, s, N, st, jlist, count_dev,
                            count_off, a_cur, b_res, p.dpart);
-    else
-        hipLaunchKernelGGL(pro_multidot_kernel<double>, dim3(2048), dim3(kBlock), 0, s, N, st, jlist, count_dev,
-                           count_off, a_cur, b_res, p.dpart);
-    hipLaunchKernelGGL(pro_reduce_kernel, dim3(256), dim3(kBlock), 0, s, N, st, count_dev, count_off, p.dpart,
-                       p.dots_a, p.dots_b);
+        hipLaunchKernelGGL(pro_reduce_kernel<float>, dim3(256), dim3(kBlock), 0, s, N, st, count_dev, count_off,
+                           p.dpart, p.dots_a, p.dots_b);
+    } else {
+        hipLaunchKernelGGL(pro_multidot_kernel<double>, dim3(kDotItems), dim3(kBlock), 0, s, N, st, jlist,
+                           count_dev, count_off, a_cur, b_res, p.dpart);
+        hipLaunchKernelGGL(pro_reduce_kernel<double>, dim3(256), dim3(kBlock), 0, s, N, st, count_dev, count_off,
+                           p.dpart, p.dots_a, p.dots_b);
+    }
     return hipGetLastError();
 }
 
