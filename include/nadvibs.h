@@ -249,8 +249,12 @@ int32_t nv_spectrum_adjoint(nv_ctx *ctx, int64_t n, const double *alpha,
  * It is the derivative of v0^T f(H) v0 in the Krylov space; it agrees with
  * central differences of the Lanczos-computed loss once the spectrum in the
  * window has converged, and it is numerically stable for any n (unlike the
- * reverse sweep of nv_lanczos_adjoint).  Needs savevecs or reorthog > 0.
- * Output layouts as nv_lanczos_adjoint.  Synchronises. */
+ * reverse sweep of nv_lanczos_adjoint).  Needs savevecs, reorthog > 0 or
+ * checkpoints (opts.checkpoint = K): then the basis is recomputed segment by
+ * segment from the checkpoints, each pair of segments (A, B >= A) once with A
+ * resident (nseg (nseg + 1) / 2 segment recomputations of K iterations, each
+ * checked bit for bit against the stored alpha/beta; 2 (K + 2) N doubles of
+ * device scratch).  Output layouts as nv_lanczos_adjoint.  Synchronises. */
 int32_t nv_krylov_gradient(nv_ctx *ctx, int64_t n, const double *tbar,
                            double *g_omega, double *g_concoef,
                            double *g_potterms);

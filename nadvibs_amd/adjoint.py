@@ -91,13 +91,15 @@ def broadened_spectrum(alpha, beta, grid, sigma, origin="lowest", e0=0.0, engine
 
 class SpectrumOfParameters(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, omega, concoef, potterms, prob, niter, grid, sigma, origin, e0, prec, reorthog, generic):
+    def forward(ctx, omega, concoef, potterms, prob, niter, grid, sigma, origin, e0, prec, reorthog, generic,
+                checkpoint):
         p = dataclasses.replace(prob, aomega=_np(omega).copy(), concoef=_np(concoef).copy(),
                                 potterms=_np(potterms).copy(), shiftref=0)
         eng = Engine(p)
         if generic:
             eng.select_hv_kernel(True)
-        eng.lanczos_init(niter, reorthog=reorthog, prec=prec, savevecs=True)
+        ck = checkpoint if reorthog == 0 else 0
+        eng.lanczos_init(niter, reorthog=reorthog, prec=prec, savevecs=ck == 0, checkpoint=ck)
         eng.run(1, niter)
         a, b, _ = eng.coefficients()
         x = _np(grid)
@@ -111,16 +113,18 @@ class SpectrumOfParameters(torch.autograd.Function):
         g = ctx.eng.krylov_gradient(tb)
         ctx.eng.close()
         return (torch.from_numpy(g["omega"]), torch.from_numpy(g["concoef"]), torch.from_numpy(g["potterms"]),
-                None, None, None, None, None, None, None, None, None)
+                None, None, None, None, None, None, None, None, None, None)
 
 
 def spectrum_of_parameters(prob: Problem, omega, concoef, potterms, niter, grid, sigma, origin="lowest", e0=0.0,
-                           prec="f64", reorthog=0, generic=True):
+                           prec="f64", reorthog=0, generic=True, checkpoint=0):
     """Broadened stick spectrum (convolve.py:28-48 without the division by the
     maximum) of the niter-step run from the default start vector, on grid
     (cm^-1), differentiable in omega, concoef and potterms through the
     Krylov-projected gradient.  reorthog > 0 runs with partial
     re-orthogonalisation; generic = True skips the run-time compile of the
-    specialised H*v (each call builds a new operator)."""
+    specialised H*v (each call builds a new operator).  checkpoint = K > 0
+    (with reorthog = 0) keeps only (q, r) every K iterations instead of the
+    basis; the backward pass then recomputes the basis segment by segment."""
     return SpectrumOfParameters.apply(omega, concoef, potterms, prob, niter, grid, sigma, origin, e0, prec, reorthog,
-                                      generic)
+                                      generic, checkpoint)

@@ -1006,6 +1006,43 @@ static void map_gradients(nv_ctx *c, const std::vector<double> &gacc, double *g_
         for (int64_t k = 0; k < op.M; ++k) g_omega[k] = gw[k];
 }
 
+// Recomputes q_{s0-1} .. q_{s1} and r_{s1} of the stored run (s0 = sg*K + 1,
+// s1 = min((sg+1)*K, n)) from checkpoint sg into seg, q_j in slot j % (K+2),
+// with the forward's own kernels, and checks the recomputed alpha/beta bit
+// for bit against the stored ones.  ta, tb: niter + 3 doubles of scratch.
+static int32_t recompute_segment(nv_ctx *c, int64_t sg, int64_t K, int64_t n, double *seg, double *ta, double *tb) {
+    const int64_t N = c->op.N, niter = c->opts.niter;
+    const int64_t s0 = sg * K + 1, s1 = std::min((sg + 1) * K, n);
+    hipStream_t s = c->stream;
+    nv::LzState ts = c->st;
+    ts.V = seg;
+    ts.nslots = K + 2;
+    ts.alpha = ta;
+    ts.beta = tb;
+    hipError_t e = hipMemcpyAsync(ta, c->st.alpha, (niter + 3) * 8, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tb, c->st.beta, (niter + 3) * 8, hipMemcpyDeviceToDevice, s);
+    const double *cpk = c->cp + (size_t)sg * 2 * N;
+    if (e == hipSuccess) e = hipMemcpyAsync(seg + ((s0 - 1) % (K + 2)) * N, cpk, N * 8, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(seg + (s0 % (K + 2)) * N, cpk + N, N * 8, hipMemcpyDeviceToDevice, s);
+    for (int64_t j = s0; j <= s1 && e == hipSuccess; ++j) {
+        e = nv::launch_set_iter(c->d_iter, j, 0, s);
+        if (e == hipSuccess) e = nv::launch_normalize(c->dev, c->opts.prec, ts, s);
+        if (e == hipSuccess) e = launch_hv_iter_st(c, ts);
+        if (e == hipSuccess) e = nv::launch_lanczos_step(c->dev, c->opts.prec, ts, s);
+    }
+    std::vector<double> ra(s1 - s0 + 1), rb(s1 - s0 + 1), oa(s1 - s0 + 1), ob(s1 - s0 + 1);
+    if (e == hipSuccess) e = hipMemcpyAsync(ra.data(), ta + s0, ra.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(rb.data(), tb + s0, rb.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(oa.data(), c->st.alpha + s0, oa.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ob.data(), c->st.beta + s0, ob.size() * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e, "checkpoint recomputation");
+    if (std::memcmp(ra.data(), oa.data(), ra.size() * 8) != 0 || std::memcmp(rb.data(), ob.data(), rb.size() * 8) != 0)
+        return fail(c, NV_ERR_STATE, "checkpoint recomputation differs from the stored run at iterations " +
+                                         std::to_string(s0) + ".." + std::to_string(s1));
+    return NV_OK;
+}
+
 int32_t nv_lanczos_adjoint(nv_ctx *c, int64_t n, const double *abar, const double *bbar, int32_t prec,
                            double *g_omega, double *g_concoef, double *g_potterms) {
     if (!c) return NV_ERR_ARG;
@@ -1071,36 +1108,8 @@ int32_t nv_lanczos_adjoint(nv_ctx *c, int64_t n, const double *abar, const doubl
         int64_t nslots = c->st.nslots;
         bool end_unnormalised = s1 == c->last_iter;
         if (!archive) {
-            // recompute q_{s0-1} .. q_{s1}, r_{s1} from checkpoint sg
-            nv::LzState ts = c->st;
-            ts.V = seg;
-            ts.nslots = K + 2;
-            ts.alpha = ta;
-            ts.beta = tb;
-            e = hipMemcpyAsync(ta, c->st.alpha, (niter + 3) * 8, hipMemcpyDeviceToDevice, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(tb, c->st.beta, (niter + 3) * 8, hipMemcpyDeviceToDevice, s);
-            const double *cpk = c->cp + (size_t)sg * 2 * N;
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(seg + ((s0 - 1) % (K + 2)) * N, cpk, N * 8, hipMemcpyDeviceToDevice, s);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(seg + (s0 % (K + 2)) * N, cpk + N, N * 8, hipMemcpyDeviceToDevice, s);
-            for (int64_t j = s0; j <= s1 && e == hipSuccess; ++j) {
-                e = nv::launch_set_iter(c->d_iter, j, 0, s);
-                if (e == hipSuccess) e = nv::launch_normalize(c->dev, c->opts.prec, ts, s);
-                if (e == hipSuccess) e = launch_hv_iter_st(c, ts);
-                if (e == hipSuccess) e = nv::launch_lanczos_step(c->dev, c->opts.prec, ts, s);
-            }
-            if (e != hipSuccess) break;
-            std::vector<double> ra(s1 - s0 + 1), rb(s1 - s0 + 1), oa(s1 - s0 + 1), ob(s1 - s0 + 1);
-            e = hipMemcpyAsync(ra.data(), ta + s0, ra.size() * 8, hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(rb.data(), tb + s0, rb.size() * 8, hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(oa.data(), c->st.alpha + s0, oa.size() * 8, hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipMemcpyAsync(ob.data(), c->st.beta + s0, ob.size() * 8, hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e != hipSuccess) break;
-            if (std::memcmp(ra.data(), oa.data(), ra.size() * 8) != 0 || std::memcmp(rb.data(), ob.data(), rb.size() * 8) != 0)
-                return done(fail(c, NV_ERR_STATE, "checkpoint recomputation differs from the stored run at iterations " +
-                                                      std::to_string(s0) + ".." + std::to_string(s1)));
+            const int32_t rc = recompute_segment(c, sg, K, n, seg, ta, tb);
+            if (rc != NV_OK) return done(rc);
             V = seg;
             nslots = K + 2;
             end_unnormalised = true;
@@ -1195,48 +1204,89 @@ int32_t nv_krylov_gradient(nv_ctx *c, int64_t n, const double *tbar, double *g_o
     if (!c) return NV_ERR_ARG;
     if (!tbar) return fail(c, NV_ERR_ARG, "tbar missing");
     if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
-    if (c->opts.reorthog <= 0 && c->opts.savevecs == 0)
-        return fail(c, NV_ERR_STATE, "the Krylov gradient needs the kept basis (savevecs or reorthog)");
+    const bool archive = c->opts.reorthog > 0 || c->opts.savevecs != 0;
+    if (!archive && !c->cp)
+        return fail(c, NV_ERR_STATE, "the Krylov gradient needs the kept basis (savevecs or reorthog) or checkpoints");
     if (n < 1 || n > c->last_iter) return fail(c, NV_ERR_ARG, "n must lie in 1..last iteration");
     NV_HIP(c, hipSetDevice(c->device));
     NV_HIP(c, hipStreamSynchronize(c->stream));
     int32_t rc = ensure_gop(c);
     if (rc != NV_OK) return rc;
-    const int64_t N = c->op.N;
+    const int64_t N = c->op.N, niter = c->opts.niter;
+    const int64_t K = archive ? n : c->opts.checkpoint;
     const int nss = (int)(c->op.ns * c->op.ns);
     const int groups = nv::grad_groups(c->gdev), chunks = nv::grad_chunks(c->gdev);
-    const int64_t jc = std::min<int64_t>(n, 4 * nv::kRitzChunk);
+    const int64_t jc = std::min<int64_t>(K, 4 * nv::kRitzChunk);
     std::vector<void *> tmp;
     auto alloc = [&](size_t bytes, void **p) -> hipError_t {
         hipError_t er = hipMalloc(p, bytes);
         if (er == hipSuccess) tmp.push_back(*p);
         return er;
     };
+    auto done = [&](int32_t code) {
+        for (void *p : tmp) hipFree(p);
+        return code;
+    };
     double *tb = nullptr, *W = nullptr, *gpart = nullptr, *gacc = nullptr;
+    double *segA = nullptr, *segB = nullptr, *ta = nullptr, *tbb = nullptr;
     hipError_t e = alloc((size_t)n * n * 8, (void **)&tb);
     if (e == hipSuccess) e = alloc((size_t)jc * N * 8, (void **)&W);
     if (e == hipSuccess) e = alloc((size_t)groups * chunks * nss * 8, (void **)&gpart);
     if (e == hipSuccess) e = alloc((size_t)groups * nss * 8, (void **)&gacc);
+    if (!archive) {
+        if (e == hipSuccess) e = alloc((size_t)2 * (K + 2) * N * 8, (void **)&segA);
+        segB = segA + (K + 2) * N;
+        if (e == hipSuccess) e = alloc((size_t)(niter + 3) * 8, (void **)&ta);
+        if (e == hipSuccess) e = alloc((size_t)(niter + 3) * 8, (void **)&tbb);
+    }
+    if (e != hipSuccess) return done(hip_fail(c, e, "krylov gradient allocation"));
     hipStream_t s = c->stream;
-    if (e == hipSuccess) e = hipMemcpyAsync(tb, tbar, (size_t)n * n * 8, hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync(tb, tbar, (size_t)n * n * 8, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemsetAsync(gacc, 0, (size_t)groups * nss * 8, s);
-    // q_1 .. q_n are slots 1..n of the archive
-    const double *Q = c->st.V + N;
-    for (int64_t j0 = 0; j0 < n && e == hipSuccess; j0 += jc) {
-        const int64_t m = std::min(jc, n - j0);
-        // w_j = sum_k Tbar[j][k] q_k for j in [j0, j0+m): Tbar is symmetric, so its rows serve as
-        // the component-major coefficient table of launch_ritz_vectors
-        e = nv::launch_ritz_vectors(Q, N, n, tb + j0, m, W, s);
-        for (int64_t j = 0; j < m && e == hipSuccess; ++j)
-            e = nv::launch_grad_contract(c->gdev, W + j * N, Q + (j0 + j) * N, gpart, gacc, s);
+    // Sum over j in [o0, o1] of q_j^T (dH/dtheta) w_j, w_j = sum_{k=i0..i1} Tbar[j][k] q_k: q_j in
+    // slot (j % ons) of obuf, q_k in slot (k % ins) of ibuf.  Tbar is symmetric, so its rows
+    // serve as the component-major coefficient table of launch_ritz_vectors.
+    auto contract = [&](const double *obuf, int64_t ons, int64_t o0, int64_t o1, const double *ibuf, int64_t ins,
+                        int64_t i0, int64_t i1) {
+        for (int64_t j0 = o0; j0 <= o1 && e == hipSuccess; j0 += jc) {
+            const int64_t m = std::min(jc, o1 - j0 + 1);
+            e = nv::launch_ritz_vectors(ibuf, N, i1 - i0 + 1, tb + (i0 - 1) * n + (j0 - 1), n, m, i0 % ins, ins, W,
+                                        s);
+            for (int64_t j = 0; j < m && e == hipSuccess; ++j)
+                e = nv::launch_grad_contract(c->gdev, W + j * N, obuf + ((j0 + j) % ons) * N, gpart, gacc, s);
+        }
+    };
+    if (archive) {
+        // q_j in slot j of the archive
+        contract(c->st.V, c->st.nslots, 1, n, c->st.V, c->st.nslots, 1, n);
+    } else {
+        // Segment pairs (A, B), B >= A, from the checkpoints: segment A stays resident while
+        // every later segment is recomputed beside it; an off-diagonal pair contributes both
+        // orders, so the sum covers every (j, k).
+        const int64_t nseg = (n + K - 1) / K;
+        for (int64_t A = 0; A < nseg && e == hipSuccess; ++A) {
+            rc = recompute_segment(c, A, K, n, segA, ta, tbb);
+            if (rc != NV_OK) return done(rc);
+            const int64_t a0 = A * K + 1, a1 = std::min((A + 1) * K, n);
+            for (int64_t B = A; B < nseg && e == hipSuccess; ++B) {
+                const int64_t b0 = B * K + 1, b1 = std::min((B + 1) * K, n);
+                if (B == A) {
+                    contract(segA, K + 2, a0, a1, segA, K + 2, a0, a1);
+                    continue;
+                }
+                rc = recompute_segment(c, B, K, n, segB, ta, tbb);
+                if (rc != NV_OK) return done(rc);
+                contract(segA, K + 2, a0, a1, segB, K + 2, b0, b1);
+                contract(segB, K + 2, b0, b1, segA, K + 2, a0, a1);
+            }
+        }
     }
     std::vector<double> gh((size_t)groups * nss);
     if (e == hipSuccess) e = hipMemcpyAsync(gh.data(), gacc, gh.size() * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    for (void *p : tmp) hipFree(p);
-    if (e != hipSuccess) return hip_fail(c, e, "krylov gradient");
+    if (e != hipSuccess) return done(hip_fail(c, e, "krylov gradient"));
     map_gradients(c, gh, g_omega, g_concoef, g_potterms);
-    return NV_OK;
+    return done(NV_OK);
 }
 
 // ---- Gaussian broadening ----------------------------------------------------

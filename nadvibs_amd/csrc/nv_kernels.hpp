@@ -85,6 +85,10 @@ constexpr int kGramT = 32;      // Gram tile edge
 // Y[i][r] = sum_k vec[k*n + i] V[k*N + r], i < nroots, k < n.
 hipError_t launch_ritz_vectors(const double *V, int64_t N, int64_t n, const double *vec, int64_t nroots, double *Y,
                                hipStream_t s);
+// The same with V_k at V + ((slot0 + k) % nslots) * N (nslots >= n) and
+// coefficients vec[k*ldv + i].
+hipError_t launch_ritz_vectors(const double *V, int64_t N, int64_t n, const double *vec, int64_t ldv, int64_t nroots,
+                               int64_t slot0, int64_t nslots, double *Y, hipStream_t s);
 // Partial-result blocks per root of launch_topk (pm, pi hold nroots * topk_blocks(N)).
 int topk_blocks(int64_t N);
 // For each of nroots vectors Y + root*N: the kmax largest squared elements in

@@ -17,10 +17,12 @@ namespace nv {
 
 // ---- Y = V S ----------------------------------------------------------------
 // Y[c0+c][r] = sum_k S(k, c0+c) V_k[r], k = 0..n-1, in increasing k (the
-// reference's GA_ADD order).  V_k at V + k*N; S component-major,
-// vec[k*n + i] = component k of root i.
+// reference's GA_ADD order).  V_k at V + ((slot0 + k) % nslots) * N (the
+// basis or a circular segment buffer); S component-major with row stride ldv,
+// vec[k*ldv + i] = component k of root i.
 __global__ __launch_bounds__(256) void ritz_vectors_kernel(const double *__restrict__ V, int64_t N, int64_t n,
-                                                          const double *__restrict__ vec, int64_t c0, int nc,
+                                                          const double *__restrict__ vec, int64_t ldv, int64_t c0,
+                                                          int nc, int64_t slot0, int64_t nslots,
                                                           double *__restrict__ Y) {
     __shared__ double s[kRitzKB][kRitzChunk];
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -32,14 +34,24 @@ __global__ __launch_bounds__(256) void ritz_vectors_kernel(const double *__restr
         __syncthreads();
         for (int t = threadIdx.x; t < kRitzKB * kRitzChunk; t += 256) {
             const int kk = t / kRitzChunk, cc = t % kRitzChunk;
-            s[kk][cc] = (kk < kb && cc < nc) ? vec[(k0 + kk) * n + c0 + cc] : 0.0;
+            s[kk][cc] = (kk < kb && cc < nc) ? vec[(k0 + kk) * ldv + c0 + cc] : 0.0;
         }
         __syncthreads();
         if (r < N) {
-            const double *vk = V + k0 * N + r;
+            // the slots of this stage: one run up to the end of the buffer, then from its start
+            const int64_t first = (slot0 + k0) % nslots;
+            const int run = (int)(nslots - first < kb ? nslots - first : kb);
+            const double *va = V + first * N + r;
 #pragma unroll 4
-            for (int kk = 0; kk < kb; ++kk) {
-                const double v = vk[(int64_t)kk * N];
+            for (int kk = 0; kk < run; ++kk) {
+                const double v = va[(int64_t)kk * N];
+#pragma unroll
+                for (int c = 0; c < kRitzChunk; ++c) acc[c] = fma(s[kk][c], v, acc[c]);
+            }
+            const double *vb = V + r - (int64_t)run * N;
+#pragma unroll 4
+            for (int kk = run; kk < kb; ++kk) {
+                const double v = vb[(int64_t)kk * N];
 #pragma unroll
                 for (int c = 0; c < kRitzChunk; ++c) acc[c] = fma(s[kk][c], v, acc[c]);
             }
@@ -193,10 +205,16 @@ __global__ __launch_bounds__(256) void gram_finish_kernel(const double *__restri
 // ---- launchers ----------------------------------------------------------------
 hipError_t launch_ritz_vectors(const double *V, int64_t N, int64_t n, const double *vec, int64_t nroots, double *Y,
                                hipStream_t s) {
+    return launch_ritz_vectors(V, N, n, vec, n, nroots, 0, n, Y, s);
+}
+
+hipError_t launch_ritz_vectors(const double *V, int64_t N, int64_t n, const double *vec, int64_t ldv, int64_t nroots,
+                               int64_t slot0, int64_t nslots, double *Y, hipStream_t s) {
+    if (n < 1 || nslots < n || slot0 < 0 || ldv < 1) return hipErrorInvalidValue;
     const unsigned nb = (unsigned)((N + 255) / 256);
     for (int64_t c0 = 0; c0 < nroots; c0 += kRitzChunk) {
         const int nc = (int)(nroots - c0 < kRitzChunk ? nroots - c0 : kRitzChunk);
-        ritz_vectors_kernel<<<nb, 256, 0, s>>>(V, N, n, vec, c0, nc, Y);
+        ritz_vectors_kernel<<<nb, 256, 0, s>>>(V, N, n, vec, ldv, c0, nc, slot0 % nslots, nslots, Y);
     }
     return hipGetLastError();
 }

@@ -169,6 +169,30 @@ def test_krylov_gradient_matches_oracle_on_the_same_basis(reorthog):
     assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_krylov_gradient_from_checkpoints_equals_kept_basis(prec):
+    """Checkpoints every 7 iterations (60 = 8 full segments + 4): the segment
+    pairs recomputed from the checkpoints give the kept-basis gradient (the
+    recomputed vectors are the stored ones bit for bit; only the summation
+    grouping differs).  Q2 operator with inter-state diagonal couplings, so
+    dH/dtheta is not symmetric for those slots."""
+    p = _prob(ns=3, nf=(5, 4, 3, 3), seed=38, quirks=3, offe=30.0)
+    n = 60
+    grid = np.linspace(-100.0, 4000.0, 400)
+    out = []
+    for kw in (dict(savevecs=True), dict(checkpoint=7)):
+        with Engine(p) as eng:
+            eng.lanczos_init(n, prec=prec, **kw)
+            eng.run(1, n)
+            a, b, _ = eng.coefficients()
+            env = eng.spectrum(a, b, grid, 30.0, "fixed", 0.0)
+            _, _, tb = eng.spectrum_adjoint(a, b, grid, 30.0, 2 * env, "fixed", 0.0, full=True)
+            out.append((a, b, _flat(eng.krylov_gradient(tb), p)))
+    (a1, b1, g1), (a2, b2, g2) = out
+    assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
+    assert np.max(np.abs(g2 - g1)) <= 1e-12 * np.max(np.abs(g1))
+
+
 def test_krylov_gradient_mixed_precision_within_1e5_of_fp64():
     p = _prob(ns=3, nf=(8, 7, 6, 5), seed=34, quirks=0)
     g64, _, _ = _spec_loss_grad(p, 120, "f64")
