@@ -169,10 +169,12 @@ def adjoint_cfg4(sync):
     with a Gaussian of sigma 15 cm^-1 on 4096 points above the lowest root;
     target = the spectrum of the couplings scaled by 1 + N(0, 0.05).
     Gradient with respect to every nadvibs.in coefficient and frequency
-    through the Krylov-projected adjoint, in FP64 and with FP32 H*v, over the
-    0-6000 cm^-1 window and over the low window whose roots have converged by
-    500 iterations (there the two precisions compute the same loss, so their
-    gradients can be compared)."""
+    through the Krylov-projected adjoint, in FP64 and with FP32 H*v, for two
+    runs: without re-orthogonalisation, the basis recomputed from checkpoints
+    every 50 iterations, and with partial re-orthogonalisation (kept basis).
+    Each over the 0-6000 cm^-1 window and over the low window whose roots have
+    converged by 500 iterations (there the two precisions compute the same
+    loss, so their gradients can be compared)."""
     import torch
     from nadvibs_amd.adjoint import spectrum_of_parameters
     from nadvibs_amd.synth import workload
@@ -185,31 +187,33 @@ def adjoint_cfg4(sync):
     scale = 1 + 0.05 * torch.from_numpy(rng.standard_normal(pt0.shape))
     res = {"workload": "cfg4: cfg2 operator, 500 iterations, sigma 15 cm^-1, 4096 points",
            "gradient": "Krylov-projected (nv_spectrum_adjoint + nv_krylov_gradient), table-driven H*v"}
+    runs = (("checkpoints_50", dict(reorthog=0, checkpoint=50)), ("pro_kept_basis", dict(reorthog=1)))
     for label, hi in (("window_0_6000", 6000.0), ("window_0_1500", 1500.0)):
         grid = torch.linspace(0.0, hi, 4096, dtype=torch.float64)
         with torch.no_grad():
-            target = spectrum_of_parameters(p, w0, c0, pt0 * scale, niter, grid, 15.0, "lowest")
-        out, grads, losses = {}, {}, {}
-        for prec in ("f64", "f32"):
-            w, c, pt = (t.clone().requires_grad_(True) for t in (w0, c0, pt0))
-            sync()
-            t0 = time.perf_counter()
-            env = spectrum_of_parameters(p, w, c, pt, niter, grid, 15.0, "lowest", prec=prec)
-            loss = torch.linalg.vector_norm(env - target)
-            t1 = time.perf_counter()
-            loss.backward()
-            t2 = time.perf_counter()
-            grads[prec] = torch.cat([w.grad.reshape(-1), c.grad.reshape(-1), pt.grad.reshape(-1)]).numpy()
-            losses[prec] = float(loss)
-            out[prec] = {"forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3), "loss": float(loss)}
-            print(f"# adjoint cfg4 {label} {prec}: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s, "
-                  f"loss {float(loss):.6g}", file=sys.stderr, flush=True)
-        d = grads["f32"] - grads["f64"]
-        out["parameters"] = int(grads["f64"].size)
-        out["nonzero_gradients"] = int(np.count_nonzero(grads["f64"]))
-        out["loss_rel_diff_f32_vs_f64"] = float(f"{abs(losses['f32'] - losses['f64']) / losses['f64']:.3e}")
-        out["grad_max_rel_diff_f32_vs_f64"] = float(f"{np.max(np.abs(d)) / np.max(np.abs(grads['f64'])):.3e}")
-        res[label] = out
+            target = spectrum_of_parameters(p, w0, c0, pt0 * scale, niter, grid, 15.0, "lowest", reorthog=1)
+        for run, kw in runs:
+            out, grads, losses = {}, {}, {}
+            for prec in ("f64", "f32"):
+                w, c, pt = (t.clone().requires_grad_(True) for t in (w0, c0, pt0))
+                sync()
+                t0 = time.perf_counter()
+                env = spectrum_of_parameters(p, w, c, pt, niter, grid, 15.0, "lowest", prec=prec, **kw)
+                loss = torch.linalg.vector_norm(env - target)
+                t1 = time.perf_counter()
+                loss.backward()
+                t2 = time.perf_counter()
+                grads[prec] = torch.cat([w.grad.reshape(-1), c.grad.reshape(-1), pt.grad.reshape(-1)]).numpy()
+                losses[prec] = float(loss)
+                out[prec] = {"forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3), "loss": float(loss)}
+                print(f"# adjoint cfg4 {label} {run} {prec}: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s, "
+                      f"loss {float(loss):.6g}", file=sys.stderr, flush=True)
+            d = grads["f32"] - grads["f64"]
+            out["parameters"] = int(grads["f64"].size)
+            out["nonzero_gradients"] = int(np.count_nonzero(grads["f64"]))
+            out["loss_rel_diff_f32_vs_f64"] = float(f"{abs(losses['f32'] - losses['f64']) / losses['f64']:.3e}")
+            out["grad_max_rel_diff_f32_vs_f64"] = float(f"{np.max(np.abs(d)) / np.max(np.abs(grads['f64'])):.3e}")
+            res[f"{label}_{run}"] = out
     return res
 
 
