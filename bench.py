@@ -217,6 +217,45 @@ def adjoint_cfg4(sync):
     return res
 
 
+def cfg3_probe(sync, iters=8):
+    """cfg3 (nstates 4, nmodes 10, basis [8,8,6,6,6,5,5,4,4,4], N = 88,473,600,
+    ordr 3, reorthog 0): the first iterations in FP64 and with FP32 H*v, and
+    the H*v kernel alone (back-to-back launches).  The 1760-step operator runs
+    through the table-driven kernel.  The full 3000-iteration job is not run
+    here; its time is the per-iteration figure times 3000."""
+    from nadvibs_amd.engine import Engine
+    from nadvibs_amd.synth import workload
+    p = workload("cfg3")
+    out = {"workload": f"cfg3: nstates={p.nstates} nmodes={p.nmodes} basis={p.nfunc} ordr={p.ordr} N={p.n}",
+           "iterations_timed": iters}
+    coef = {}
+    with Engine(p, device=0) as eng:
+        out["hv_kernel"] = eng.hv_kernel("f32")
+        flops = eng.hv_flops()
+        out["hv_flops"] = flops
+        for prec, el in (("f64", 8), ("f32", 4)):
+            eng.lanczos_init(niter=iters, reorthog=0, prec=prec)
+            eng.run(1, 1)
+            sync()
+            t0 = time.perf_counter()
+            eng.run(2, iters)
+            a, b, _ = eng.coefficients()
+            dt = (time.perf_counter() - t0) / (iters - 1)
+            coef[prec] = (a, b)
+            eng.lanczos_init(niter=4, reorthog=0, prec=prec)
+            eng.update_lanczos(1)
+            us = eng.profile_hv(5)
+            out[prec] = {"ms_per_iteration": round(dt * 1e3, 2), "hv_avg_launch_us": round(us, 1),
+                         "hv_gbps": round(p.n * 2 * el / us / 1e3, 1),
+                         "hv_tflops": round(flops / us / 1e6, 2)}
+            print(f"# cfg3 {prec}: {dt * 1e3:.1f} ms/iteration, H*v {us:.0f} us", file=sys.stderr, flush=True)
+    a64, b64 = coef["f64"]
+    a32, b32 = coef["f32"]
+    out["alpha_max_rel_diff_f32_vs_f64"] = float(f"{np.max(np.abs(a32 - a64) / np.abs(a64)):.3e}")
+    out["speedup_f32_vs_f64"] = round(out["f64"]["ms_per_iteration"] / out["f32"]["ms_per_iteration"], 3)
+    return out
+
+
 def load_traffic(path):
     """HBM bytes per H*v launch measured with rocprofv3 counters, if a
     summary with a 'traffic_bytes_per_launch:' line exists."""
@@ -266,6 +305,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-problem batched workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the spectrum-gradient workload")
+    ap.add_argument("--no-cfg3", action="store_true", help="skip the 88.5M-row ordr-3 probe")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only the calibration copy and H*v launches, for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -309,6 +349,16 @@ def main():
             burst[p] = eng.profile_hv(200)
         print(f"# H*v back-to-back: {burst}", file=sys.stderr, flush=True)
         hv_flops = eng.hv_flops()
+        # the same job without re-orthogonalisation: H*v and the recurrence only
+        plain = {}
+        for p in ("f32", "f64"):
+            run_job(eng, args.niter, p, 0, args.chkorthog, args.seed)
+            sync()
+            t0 = time.perf_counter()
+            run_job(eng, args.niter, p, 0, args.chkorthog, args.seed)
+            sync()
+            plain[p] = args.niter / (time.perf_counter() - t0)
+        print(f"# reorthog 0: {plain}", file=sys.stderr, flush=True)
 
     iters = args.steps * args.niter
     value = iters / dt32
@@ -348,10 +398,15 @@ def main():
         rec["max_abs_dI"] = float(f"{dI:.3e}")
         rec["n_converged_fp64"] = nconv
         rec["error_target_met"] = bool(dE <= 1e-3)
+    rec["reorthog0"] = {"f32_iter_per_s": round(plain["f32"], 1), "f64_iter_per_s": round(plain["f64"], 1),
+                        "speedup_f32_vs_f64": round(plain["f32"] / plain["f64"], 3),
+                        "note": "same job without partial re-orthogonalisation (ghost copies are not compared)"}
     if not args.no_batch:
         rec["batched_cfg5"] = batched_cfg5(sync, 0.0 if args.no_cpu else args.cpu_seconds)
     if not args.no_adjoint:
         rec["adjoint_cfg4"] = adjoint_cfg4(sync)
+    if not args.no_cfg3:
+        rec["cfg3"] = cfg3_probe(sync)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
     print(json.dumps(rec), flush=True)
