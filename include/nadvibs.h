@@ -48,7 +48,10 @@ extern "C" {
 /* ---- H*v precision ----------------------------------------------------- */
 #define NV_PREC_F64 0    /* off-diagonal couplings in FP64 */
 #define NV_PREC_F32 1    /* off-diagonal couplings in FP32, diagonal FP64 */
-#define NV_PREC_BF16 2   /* vector streamed as BF16 hi+lo pair, FP32 arithmetic, diagonal FP64 */
+#define NV_PREC_BF16 2   /* off-diagonal couplings applied to a BF16 copy of the
+                            vector (8-bit mantissa) in FP32 arithmetic, diagonal
+                            FP64: 2^-9 relative input rounding, for bandwidth
+                            studies, not for the 1e-3 cm^-1 accuracy target */
 
 /* ---- problem description ---------------------------------------------- */
 /* The data of basis.in / nadvibs.in after reading, before the reference's

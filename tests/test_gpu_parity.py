@@ -113,6 +113,23 @@ def test_lanczos_coefficients_cfg1(prec, tol, hv_kernel):
     np.testing.assert_allclose(b[:k], ref["beta"][:k], rtol=tol * 10)
 
 
+def test_lanczos_bf16_input_tracks_fp64():
+    """BF16 copy of the H*v input (8-bit mantissa, FP32 arithmetic, FP64
+    diagonal and recurrence): the first coefficients follow the FP64 oracle
+    to the BF16 rounding level (2^-9 relative per element), and no further.
+    The mode trades accuracy for half the FP32 vector bytes."""
+    p = workload("cfg1")
+    ref = Oracle(p).lanczos(40)
+    with Engine(p) as eng:
+        eng.lanczos_init(niter=40, prec="bf16")
+        eng.run(1, 40)
+        a, b, _ = eng.coefficients()
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(b))
+    rel = np.abs(a[:10] - ref["alpha"][:10]) / np.abs(ref["alpha"][:10])
+    assert rel.max() < 1e-2
+    assert rel.max() > 1e-7  # it is not running the FP32 path
+
+
 def test_ritz_matches_givens():
     p = workload("cfg1")
     ref = Oracle(p).lanczos(300)
