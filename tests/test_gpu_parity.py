@@ -208,6 +208,31 @@ def test_pro_with_noise_keeps_orthogonality():
     assert np.max(np.abs(Q @ Q.T - np.eye(150))) < 1e-6
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("nf", [[5, 3, 3], [7, 9, 11]], ids=["N135", "N2079"])
+def test_pro_odd_dimension_keeps_orthogonality(prec, nf):
+    """Odd N (the FP32 basis copy is read in element pairs with a zero pad
+    element per slot) at one and at two dot segments, both precisions:
+    re-orthogonalisation happens and keeps the basis orthogonal.  FP64: below
+    1e-6.  Mixed: FP32 rounding of H*v regrows the overlaps between the checks
+    every 10 iterations; the oracle's emulation reaches 1.2e-5 on the 135-row
+    problem (67 iterations, half the dimension) and 2.4e-7 on the 2079-row one,
+    so the bound is 5e-5."""
+    p = make_problem(3, nf, 2, (400, 3000), 250.0, 60.0, 40.0, seed=21)
+    n = min(100, p.n // 2)
+    with Engine(p) as eng:
+        eng.lanczos_init(niter=n, reorthog=1, chkorthog=20, prec=prec, seed=3)
+        eng.run(1, n)
+        a, b, _ = eng.coefficients()
+        Q = np.array([eng.lanczos_vector(j) for j in range(1, n + 1)])
+        events, touched = eng.pro_stats()
+    print(f"N {p.n} {prec}: events {events} touched {touched}")
+    assert p.n % 2 == 1
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(b))
+    assert events > 0
+    assert np.max(np.abs(Q @ Q.T - np.eye(n))) < (1e-6 if prec == "f64" else 5e-5)
+
+
 def test_exact_omega_mode():
     p = workload("cfg1")
     ref = Oracle(p).lanczos(80, reorthog=2)
