@@ -119,6 +119,13 @@ int32_t nv_hv_device(nv_ctx *ctx, int32_t prec, const double *x_dev,
 int32_t nv_start_vector(nv_ctx *ctx, double *q);
 
 /* ---- Lanczos (replaces the loop of Source/main.f90:63-71) ---------------- */
+/* Re-orthogonalisation thresholds of the mixed modes (pro_eps = 0): a vector
+ * is re-orthogonalised when an overlap estimate reaches NV_PRO_MIXED_TAU, over
+ * the range out to estimates below NV_PRO_MIXED_ETA (FP64 H*v: sqrt(eps) and
+ * eps^(3/4), Source/top.f90:922-950). */
+#define NV_PRO_MIXED_TAU 5.9604644775390625e-08   /* 2^-24 */
+#define NV_PRO_MIXED_ETA 3.7252902984619141e-09   /* 2^-28 */
+
 typedef struct nv_lanczos_opts {
     int64_t niter;            /* total iterations (basis.in niter) */
     int32_t reorthog;         /* basis.in reorthog: 0 none, 1 partial with the
@@ -126,9 +133,12 @@ typedef struct nv_lanczos_opts {
     int32_t prec;             /* NV_PREC_* for H*v */
     int64_t chkorthog;        /* exact overlap check period (default 100); with
                                  NV_PREC_F32/BF16 and pro_eps = 0 the period is
-                                 min(chkorthog, 10): the omega model then assumes
-                                 FP64 rounding of H*v and the checks keep it
-                                 honest */
+                                 min(chkorthog, 10) and the thresholds are
+                                 NV_PRO_MIXED_TAU / NV_PRO_MIXED_ETA: the omega
+                                 model assumes FP64 rounding of H*v, the checks
+                                 keep it honest, and the thresholds sit above
+                                 the overlap level that FP32 rounding alone
+                                 produces (~1e-8) */
     uint64_t seed;            /* seed of the omega-estimate noise */
     double noise_scale;       /* multiplies every noise draw (1 = reference, 0 = off) */
     double pro_eps;           /* unit roundoff assumed by the omega recurrence and

@@ -487,15 +487,26 @@ __global__ __launch_bounds__(kBlock) void pro_beta_kernel(ProDev p, LzState st, 
     if (threadIdx.x == 0) st.beta[*st.iter] = sqrt(s);
 }
 
-// Exact-check period: the reference's chkorthog; in the mixed modes at most
-// 10, since the omega recurrence models FP64 rounding of H*v and the FP32
-// couplings lose orthogonality faster between checks.
-// Mixed modes with the default (FP64) roundoff model check the overlaps every
-// min(chkorthog, 10) iterations; a caller that sets pro_eps chooses the period.
+// Mixed modes with the default (FP64) roundoff model: FP32 rounding of H*v
+// alone keeps the overlaps near 1e-8, at sqrt(2^-53) itself, so the omega
+// recurrence (which models FP64 rounding) misses the loss between checks and
+// every check would re-orthogonalise against the whole basis.  These modes
+// check every min(chkorthog, 10) iterations and use thresholds above that
+// level (NV_PRO_MIXED_TAU, NV_PRO_MIXED_ETA).  A caller that sets pro_eps
+// chooses the period and gets sqrt(eps), eps^(3/4).
+static bool mixed_model(const nv_lanczos_opts &o) { return o.prec != NV_PREC_F64 && !(o.pro_eps > 0); }
+
 static int64_t chk_period(const nv_lanczos_opts &o) {
     int64_t c = o.chkorthog < 1 ? 100 : o.chkorthog;
-    if (o.prec != NV_PREC_F64 && !(o.pro_eps > 0) && c > 10) c = 10;
+    if (mixed_model(o) && c > 10) c = 10;
     return c;
+}
+
+// Thresholds and roundoff of the omega model (Source/top.f90:862-874, :922-950).
+static void pro_thresholds(ProDev &p, const nv_lanczos_opts &o) {
+    p.eps = o.pro_eps > 0 ? o.pro_eps : PRO_EPS64;
+    p.seps = mixed_model(o) ? NV_PRO_MIXED_TAU : sqrt(p.eps);
+    p.eta = mixed_model(o) ? NV_PRO_MIXED_ETA : pow(p.eps, 0.75);
 }
 
 hipError_t pro_init(ProState &ps, const nv_lanczos_opts &o, const Operator &op, const NvOpDev &dev,
@@ -507,9 +518,7 @@ hipError_t pro_init(ProState &ps, const nv_lanczos_opts &o, const Operator &op, 
     p.exact = o.reorthog > 1;
     p.seed = o.seed;
     p.noise = o.noise_scale;
-    p.eps = o.pro_eps > 0 ? o.pro_eps : PRO_EPS64;
-    p.seps = sqrt(p.eps);
-    p.eta = pow(p.eps, 0.75);
+    pro_thresholds(p, o);
     p.epsn_scale = (double)op.ns * (double)op.D;
     p.epsn = p.eps * p.epsn_scale;
     p.ostride = o.niter + 3;
@@ -549,9 +558,7 @@ hipError_t pro_reset(ProState &ps, const nv_lanczos_opts &o, hipStream_t s) {
     p.chkorthog = chk_period(o);
     p.seed = o.seed;
     p.noise = o.noise_scale;
-    p.eps = o.pro_eps > 0 ? o.pro_eps : PRO_EPS64;
-    p.seps = sqrt(p.eps);
-    p.eta = pow(p.eps, 0.75);
+    pro_thresholds(p, o);
     p.epsn = p.eps * (double)p.epsn_scale;
     hipError_t e = hipMemsetAsync(p.omega, 0, 2 * p.ostride * sizeof(double), s);
     if (e == hipSuccess) e = hipMemsetAsync(p.lo, 0, p.cap * sizeof(int64_t), s);

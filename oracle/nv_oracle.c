@@ -330,6 +330,10 @@ static double g_eps_pro = 0.0;
 /* Unit roundoff used by the omega recurrence and its thresholds (0: the
  * reference's 2^-53). */
 void or_set_eps_pro(double e) { g_eps_pro = e; }
+/* Thresholds of the detection (0 = sqrt(eps), eps^(3/4)): the library's
+ * mixed modes use NV_PRO_MIXED_TAU / NV_PRO_MIXED_ETA. */
+static double g_tau = 0.0, g_eta = 0.0;
+void or_set_pro_thresholds(double tau, double eta) { g_tau = tau; g_eta = eta; }
 /* Emulation switch: the off-diagonal (vibrational) part of H*v with the
  * input rounded to float and float accumulation, the diagonal in double --
  * the precision split of the library's mixed mode. */
@@ -652,7 +656,8 @@ int or_lanczos(const OOp *op, long niter, int reorthog, long chkorthog, uint64_t
                long *stats) {
     const long N = op->N;
     const double eps = g_eps_pro > 0 ? g_eps_pro : 1.1102230246251565e-16;  /* determine_epsilon: 2^-53 */
-    const double eta = pow(eps, 0.75);
+    const double eta = g_eta > 0 ? g_eta : pow(eps, 0.75);
+    const double tau = g_tau > 0 ? g_tau : sqrt(eps);
     const int orthog = reorthog > 0, exact = reorthog > 1;
     double *q1 = (double *)calloc(N, sizeof(double));
     double *q2 = (double *)calloc(N, sizeof(double));
@@ -732,7 +737,7 @@ int or_lanczos(const OOp *op, long niter, int reorthog, long chkorthog, uint64_t
             northog = 0;
             long plo = 0, pro = 0;
             for (long i = 1; i <= it; ++i) {
-                if (!(fabs(om2[i]) >= sqrt(eps))) continue;
+                if (!(fabs(om2[i]) >= tau)) continue;
                 long l = i;
                 while (l >= 1 && !(fabs(om2[l]) < eta)) --l;
                 if (l < 1) l = 1;

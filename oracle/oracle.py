@@ -54,6 +54,7 @@ def lib():
                                  ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long)]
         l.or_set_off32.argtypes = [ctypes.c_int]
         l.or_set_eps_pro.argtypes = [ctypes.c_double]
+        l.or_set_pro_thresholds.argtypes = [ctypes.c_double, ctypes.c_double]
         l.or_givens.argtypes = [ctypes.c_long, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int), _dp]
         _lib = l
     return _lib
@@ -126,7 +127,8 @@ class Oracle:
         """mixed=True emulates the library's mixed mode: off-diagonal part of
         H*v from a float copy with float accumulation; with the default
         pro_eps (0: 2^-53) exact overlap checks every min(chkorthog, 10)
-        iterations.  pro_eps > 0 sets the roundoff of the omega model."""
+        iterations and the thresholds MIXED_TAU / MIXED_ETA.  pro_eps > 0 sets
+        the roundoff of the omega model (thresholds sqrt(eps), eps^(3/4))."""
         if pro_eps:
             set_eps_pro(pro_eps)
             try:
@@ -137,10 +139,12 @@ class Oracle:
             lib().or_set_off32(1)
             if not lib_eps_set():
                 chkorthog = min(chkorthog, 10)
+                lib().or_set_pro_thresholds(MIXED_TAU, MIXED_ETA)
             try:
                 return self.lanczos(niter, reorthog, chkorthog, seed, noise, start, keep_basis, mixed=False)
             finally:
                 lib().or_set_off32(0)
+                lib().or_set_pro_thresholds(0.0, 0.0)
         a = np.zeros(niter)
         b = np.zeros(niter + 1)
         om = np.zeros(2 * (niter + 1))
@@ -164,6 +168,9 @@ def set_off32(on):
 
 
 _EPS_PRO = [0.0]
+# NV_PRO_MIXED_TAU / NV_PRO_MIXED_ETA of include/nadvibs.h
+MIXED_TAU = 2.0 ** -24
+MIXED_ETA = 2.0 ** -28
 
 
 def set_eps_pro(eps):

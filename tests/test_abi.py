@@ -56,3 +56,13 @@ def test_null_arguments():
     assert lib.nv_create(None, 0, ctypes.byref(h)) == 1
     assert lib.nv_destroy(None) == 0
     assert lib.nv_dims(None, None, None) == 1
+
+
+def test_mixed_thresholds_match_between_header_and_oracle():
+    """The oracle's emulation of the mixed mode uses the header's thresholds."""
+    import re
+    from oracle.oracle import MIXED_ETA, MIXED_TAU
+    text = open(os.path.join(ROOT, "include", "nadvibs.h")).read()
+    tau = float(re.search(r"#define NV_PRO_MIXED_TAU (\S+)", text).group(1))
+    eta = float(re.search(r"#define NV_PRO_MIXED_ETA (\S+)", text).group(1))
+    assert tau == MIXED_TAU and eta == MIXED_ETA
