@@ -148,6 +148,7 @@ program nadvibs_main
     ! savevecs (Source/secondary.f90:863): the root analysis reads every Lanczos vector
     opts%savevecs = 0
     opts%checkpoint = 0
+    opts%pro_tau = 0d0; opts%pro_eta = 0d0
     if (idroots > 0 .or. soroots > 0) opts%savevecs = 1
     if (restart /= 0 .and. opts%savevecs /= 0) &
         stop 'nadvibs: idroots/soroots need every Lanczos vector and cannot follow a restart'

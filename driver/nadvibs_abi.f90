@@ -25,6 +25,8 @@ module nadvibs_abi
         real(c_double) :: noise_scale, pro_eps
         integer(c_int32_t) :: savevecs
         integer(c_int64_t) :: checkpoint
+        real(c_double) :: pro_tau
+        real(c_double) :: pro_eta
     end type nv_lanczos_opts
 
     interface

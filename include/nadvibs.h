@@ -123,8 +123,8 @@ int32_t nv_start_vector(nv_ctx *ctx, double *q);
  * is re-orthogonalised when an overlap estimate reaches NV_PRO_MIXED_TAU, over
  * the range out to estimates below NV_PRO_MIXED_ETA (FP64 H*v: sqrt(eps) and
  * eps^(3/4), Source/top.f90:922-950). */
-#define NV_PRO_MIXED_TAU 5.9604644775390625e-08   /* 2^-24 */
-#define NV_PRO_MIXED_ETA 3.7252902984619141e-09   /* 2^-28 */
+#define NV_PRO_MIXED_TAU 9.5367431640625e-07      /* 2^-20 */
+#define NV_PRO_MIXED_ETA 1.4551915228366852e-11   /* 2^-36 */
 
 typedef struct nv_lanczos_opts {
     int64_t niter;            /* total iterations (basis.in niter) */
@@ -153,6 +153,11 @@ typedef struct nv_lanczos_opts {
                                  (q_{cK}, r_{cK}) every K iterations so that
                                  nv_lanczos_adjoint can recompute the basis
                                  segment by segment; 0 = none */
+    double pro_tau;           /* > 0: re-orthogonalisation trigger on |omega|
+                                 instead of the default (sqrt(eps), or
+                                 NV_PRO_MIXED_TAU in the mixed modes) */
+    double pro_eta;           /* > 0: range end on |omega| instead of the
+                                 default (eps^(3/4), or NV_PRO_MIXED_ETA) */
 } nv_lanczos_opts;
 
 /* Allocates the Krylov basis for opts->niter iterations, builds the start

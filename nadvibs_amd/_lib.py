@@ -41,6 +41,7 @@ class NvLanczosOpts(ctypes.Structure):
         ("niter", ctypes.c_int64), ("reorthog", ctypes.c_int32), ("prec", ctypes.c_int32),
         ("chkorthog", ctypes.c_int64), ("seed", ctypes.c_uint64), ("noise_scale", ctypes.c_double),
         ("pro_eps", ctypes.c_double), ("savevecs", ctypes.c_int32), ("checkpoint", ctypes.c_int64),
+        ("pro_tau", ctypes.c_double), ("pro_eta", ctypes.c_double),
     ]
 
 

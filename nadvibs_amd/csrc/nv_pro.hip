@@ -505,8 +505,8 @@ static int64_t chk_period(const nv_lanczos_opts &o) {
 // Thresholds and roundoff of the omega model (Source/top.f90:862-874, :922-950).
 static void pro_thresholds(ProDev &p, const nv_lanczos_opts &o) {
     p.eps = o.pro_eps > 0 ? o.pro_eps : PRO_EPS64;
-    p.seps = mixed_model(o) ? NV_PRO_MIXED_TAU : sqrt(p.eps);
-    p.eta = mixed_model(o) ? NV_PRO_MIXED_ETA : pow(p.eps, 0.75);
+    p.seps = o.pro_tau > 0 ? o.pro_tau : (mixed_model(o) ? NV_PRO_MIXED_TAU : sqrt(p.eps));
+    p.eta = o.pro_eta > 0 ? o.pro_eta : (mixed_model(o) ? NV_PRO_MIXED_ETA : pow(p.eps, 0.75));
 }
 
 hipError_t pro_init(ProState &ps, const nv_lanczos_opts &o, const Operator &op, const NvOpDev &dev,

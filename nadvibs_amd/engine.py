@@ -111,10 +111,10 @@ class Engine:
 
     # -- Lanczos ---------------------------------------------------------------
     def lanczos_init(self, niter, reorthog=0, prec="f64", chkorthog=100, seed=12345, noise=1.0, pro_eps=0.0,
-                     savevecs=False, checkpoint=0):
+                     savevecs=False, checkpoint=0, pro_tau=0.0, pro_eta=0.0):
         o = _lib.NvLanczosOpts(niter=niter, reorthog=reorthog, prec=_lib.PREC[prec], chkorthog=chkorthog,
                                seed=seed, noise_scale=noise, pro_eps=pro_eps, savevecs=1 if savevecs else 0,
-                               checkpoint=checkpoint)
+                               checkpoint=checkpoint, pro_tau=pro_tau, pro_eta=pro_eta)
         self._check(self.lib.nv_lanczos_init(self.h, ctypes.byref(o)))
         self.niter = niter
         self.reorthog = reorthog

@@ -169,8 +169,8 @@ def set_off32(on):
 
 _EPS_PRO = [0.0]
 # NV_PRO_MIXED_TAU / NV_PRO_MIXED_ETA of include/nadvibs.h
-MIXED_TAU = 2.0 ** -24
-MIXED_ETA = 2.0 ** -28
+MIXED_TAU = 2.0 ** -20
+MIXED_ETA = 2.0 ** -36
 
 
 def set_eps_pro(eps):

@@ -66,3 +66,28 @@ def test_mixed_thresholds_match_between_header_and_oracle():
     tau = float(re.search(r"#define NV_PRO_MIXED_TAU (\S+)", text).group(1))
     eta = float(re.search(r"#define NV_PRO_MIXED_ETA (\S+)", text).group(1))
     assert tau == MIXED_TAU and eta == MIXED_ETA
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """ctypes mirrors of nv_problem and nv_lanczos_opts have the C layout
+    (offset of every field and the size), checked with the C compiler."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "nadvibs.h"', "int main(void) {"]
+    for cname, cls in (("nv_problem", _lib.NvProblem), ("nv_lanczos_opts", _lib.NvLanczosOpts)):
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = dict((" ".join(l.split()[:2]), int(l.split()[2])) for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    for cname, cls in (("nv_problem", _lib.NvProblem), ("nv_lanczos_opts", _lib.NvLanczosOpts)):
+        assert got[f"{cname} size"] == ctypes.sizeof(cls)
+        for f, _ in cls._fields_:
+            assert got[f"{cname} {f}"] == getattr(cls, f).offset, (cname, f)
