@@ -208,8 +208,10 @@ def test_mixed_mode_needs_frequent_exact_checks():
     """The mixed mode's rule, pinned on the emulation (off-diagonal H*v in
     float, diagonal in double): with the reference's check period of 100 the
     omega model, which assumes FP64 rounding, lets orthogonality collapse
-    (ghost 2-cycles, then overflow); with checks every 10 iterations the
-    converged eigenvalues stay within 1e-3 cm^-1 of the FP64 run."""
+    (ghost 2-cycles, then overflow); with checks every 10 iterations and the
+    mixed thresholds (MIXED_TAU, MIXED_ETA) the converged eigenvalues stay
+    within 1e-3 cm^-1 of the FP64 run for 1000 iterations (half the
+    dimension)."""
     from oracle.oracle import set_off32
     p = workload("cfg1")
     o = Oracle(p)
