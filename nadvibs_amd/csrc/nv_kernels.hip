@@ -617,17 +617,52 @@ __device__ int64_t sturm_below(const double *d, const double *e2, int64_t n, dou
     return n - nonneg;
 }
 
-__global__ void tri_bisect_kernel(int64_t n, const double *d, const double *e2, double *root) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// One wave per root: multisection.  Each round the 64 lanes take Sturm counts
+// at 64 interior points of [lo, hi] and the wave keeps the sub-interval where
+// the count reaches i + 1 (count(lo) < i + 1 <= count(hi) throughout), so a
+// round shrinks the interval 65-fold where bisection halves it.  When the
+// points no longer fall strictly inside (a few ulps left) it bisects, and it
+// stops at adjacent doubles; the root is hi, as in the bisection of GIVENS
+// (Source/yarkony.f90:345-400).  T (d, e2) is staged in LDS when it fits
+// (lds = 1): every Sturm step reads it.
+constexpr int64_t kTriLdsMax = 4000;  // 2 * 4000 doubles, under 64 KB
+
+__global__ __launch_bounds__(64) void tri_bisect_kernel(int64_t n, const double *dg, const double *e2g,
+                                                         double *root, int lds) {
+    extern __shared__ double tri_lds[];
+    const double *d = dg, *e2 = e2g;
+    if (lds) {
+        for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+            tri_lds[k] = dg[k];
+            tri_lds[n + k] = e2g[k];
+        }
+        __syncthreads();
+        d = tri_lds;
+        e2 = tri_lds + n;
+    }
+    const int64_t i = blockIdx.x;
+    const int lane = threadIdx.x;
     double lo = -1.0, hi = 1.0;
-    for (int guard = 0; guard < 2200; ++guard) {
+    for (int guard = 0; guard < 4096; ++guard) {
+        const double x = lo + (hi - lo) * ((double)(lane + 1) / 65.0);
+        const bool inside = x > lo && x < hi;
+        const bool ge = inside ? sturm_below(d, e2, n, x) >= i + 1 : x >= hi;
+        const unsigned long long ball = __ballot(ge);
+        const double xprev = __shfl_up(x, 1, 64);
+        const int k = ball ? __ffsll(ball) - 1 : 64;  // first point at or above the root
+        const double new_hi = k < 64 ? __shfl(x, k, 64) : hi;
+        const double new_lo = k == 0 ? lo : __shfl(k < 64 ? xprev : x, k < 64 ? k : 63, 64);
+        if ((new_lo > lo && new_lo < new_hi) || new_hi < hi) {
+            lo = new_lo > lo ? new_lo : lo;
+            hi = new_hi;
+            continue;
+        }
         const double mid = (lo + hi) * 0.5;
         if (mid == lo || mid == hi) break;
         if (sturm_below(d, e2, n, mid) >= i + 1) hi = mid;
         else lo = mid;
     }
-    root[i] = hi;
+    if (lane == 0) root[i] = hi;
 }
 
 // Back substitution through the pivoted factors of one root (Source/yarkony.f90:790-780).
@@ -797,7 +832,9 @@ hipError_t launch_tridiag(int64_t n, const TriWork &w, double anorm, hipStream_t
     (void)anorm;
     const int tb = 64;
     const int g = (int)((n + tb - 1) / tb);
-    hipLaunchKernelGGL(tri_bisect_kernel, dim3(g), dim3(tb), 0, s, n, w.d, w.e2, w.root);
+    const int lds = n <= kTriLdsMax;
+    hipLaunchKernelGGL(tri_bisect_kernel, dim3((unsigned)n), dim3(64), lds ? (size_t)2 * n * sizeof(double) : 0, s,
+                       n, w.d, w.e2, w.root, lds);
     hipLaunchKernelGGL(tri_cluster_kernel, dim3(1), dim3(64), 0, s, n, w.root, w.ia, w.rank);
     hipLaunchKernelGGL(tri_inverse_kernel, dim3(g), dim3(tb), 0, s, n, w.d, w.e, w.root, w.ia, w.rank, w.lu, w.vec);
     hipLaunchKernelGGL(tri_finish_kernel, dim3((unsigned)n), dim3(256), 0, s, n, w.ia, w.vec);
