@@ -212,6 +212,77 @@ __device__ __forceinline__ void offdiag_accumulate(const NvOpDev &op, const TIn 
     }
 }
 
+// The same for two rows of one thread (r0 and r0 + kBlock, digits at dig0 and
+// dig1): the step descriptors are read once for both, and the two rows'
+// table and vector loads are independent, so they overlap.  A row that is
+// not live (past the end) has zero digits and is skipped.
+template <int NS, typename TIn, typename TAcc>
+__device__ __forceinline__ void offdiag_accumulate2(const NvOpDev &op, const TIn *__restrict__ x, int r0, bool l0,
+                                                    bool l1, const uint8_t *dig0, const uint8_t *dig1,
+                                                    const TAcc *ftab, int tid, TAcc (&acc0)[NS], TAcc (&acc1)[NS]) {
+    const TAcc *__restrict__ cm = cmat_of<TAcc>(op);
+    const int r1 = r0 + kBlock;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        acc0[s] = (TAcc)0;
+        acc1[s] = (TAcc)0;
+    }
+    for (int t = 0; t < op.nterms; ++t) {
+        const NvTerm tm = op.terms[t];
+        TAcc w0[NS], w1[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            w0[s] = (TAcc)0;
+            w1[s] = (TAcc)0;
+        }
+        for (int q = 0; q < tm.nsteps; ++q) {
+            const NvStep st = op.steps[tm.first + q];
+            TAcc h0 = ftab[st.code[0] * op.nmax + dig0[st.mode[0] * kBlock + tid]];
+            TAcc h1 = ftab[st.code[0] * op.nmax + dig1[st.mode[0] * kBlock + tid]];
+#pragma unroll
+            for (int j = 1; j < NV_MAXF; ++j)
+                if (j < st.nf) {
+                    h0 *= ftab[st.code[j] * op.nmax + dig0[st.mode[j] * kBlock + tid]];
+                    h1 *= ftab[st.code[j] * op.nmax + dig1[st.mode[j] * kBlock + tid]];
+                }
+            if (l0 && h0 != (TAcc)0) {
+                const int idx = r0 + st.stride;
+                if constexpr (std::is_same<TIn, double>::value) {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) w0[s] += h0 * load_acc<TIn, TAcc>(x + (size_t)s * op.D + idx);
+                } else {
+                    const TIn *xr = x + (size_t)idx * nv_lo_stride(NS);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) w0[s] += h0 * load_acc<TIn, TAcc>(xr + s);
+                }
+            }
+            if (l1 && h1 != (TAcc)0) {
+                const int idx = r1 + st.stride;
+                if constexpr (std::is_same<TIn, double>::value) {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) w1[s] += h1 * load_acc<TIn, TAcc>(x + (size_t)s * op.D + idx);
+                } else {
+                    const TIn *xr = x + (size_t)idx * nv_lo_stride(NS);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) w1[s] += h1 * load_acc<TIn, TAcc>(xr + s);
+                }
+            }
+        }
+        const TAcc *C = cm + (size_t)t * NS * NS;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            TAcc a0 = acc0[s], a1 = acc1[s];
+#pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2) {
+                a0 += C[s * NS + s2] * w0[s2];
+                a1 += C[s * NS + s2] * w1[s2];
+            }
+            acc0[s] = a0;
+            acc1[s] = a1;
+        }
+    }
+}
+
 template <typename TAcc>
 __device__ __forceinline__ TAcc *stage_tables(const NvOpDev &op, uint8_t *&dig) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -279,6 +350,59 @@ __global__ __launch_bounds__(kBlock) void hv_iter_kernel(NvOpDev op, LzState st)
     }
     const double b = block_sum(part);
     if (tid == 0) st.partials[blockIdx.x] = b;
+}
+
+// hv_iter_kernel with two rows per thread: block b covers the rows of the
+// one-row kernel's blocks 2b and 2b+1 and writes their two partials, so alpha
+// is the same sum in the same order.
+template <int NS, typename TIn, typename TAcc>
+__global__ __launch_bounds__(kBlock) void hv_iter2_kernel(NvOpDev op, LzState st) {
+    uint8_t *dig;
+    const TAcc *ftab = stage_tables<TAcc>(op, dig);
+    uint8_t *dig1 = dig + op.M * kBlock;
+    const int tid = threadIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = 2 * b * kBlock + tid, r1 = r0 + kBlock;
+    const bool l0 = r0 < op.D, l1 = r1 < op.D;
+    const int64_t it = *st.iter;
+    const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;
+    const TIn *x = st.vlo ? reinterpret_cast<const TIn *>(st.vlo) : reinterpret_cast<const TIn *>(q);
+    if (l0) digits_energy(op, r0, dig, tid);
+    else for (int k = 0; k < op.M; ++k) dig[k * kBlock + tid] = 0;
+    if (l1) digits_energy(op, r1, dig1, tid);
+    else for (int k = 0; k < op.M; ++k) dig1[k * kBlock + tid] = 0;
+    __syncthreads();
+    double part0 = 0.0, part1 = 0.0;
+    if (l0) {
+        TAcc acc0[NS], acc1[NS];
+        offdiag_accumulate2<NS, TIn, TAcc>(op, x, r0, l0, l1, dig, dig1, ftab, tid, acc0, acc1);
+        TAcc *y = reinterpret_cast<TAcc *>(st.y);
+        double v[NS], dv[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r0];
+        diag_apply<NS>(op, r0, dig, tid, v, dv);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            y[(size_t)s * op.D + r0] = acc0[s];
+            part0 += v[s] * (dv[s] + (double)acc0[s]);
+        }
+        if (l1) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r1];
+            diag_apply<NS>(op, r1, dig1, tid, v, dv);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                y[(size_t)s * op.D + r1] = acc1[s];
+                part1 += v[s] * (dv[s] + (double)acc1[s]);
+            }
+        }
+    }
+    const double s0 = block_sum(part0);
+    const double s1 = block_sum(part1);
+    if (tid == 0) {
+        st.partials[2 * b] = s0;
+        st.partials[2 * b + 1] = s1;
+    }
 }
 
 // lanczos_step (Source/top.f90:820-842): r = (H q_i - alpha q_i) - beta(i-1) q_{i-1}
@@ -381,7 +505,7 @@ hipError_t launch_finish_sum(const double *partials, int nparts, double *out, co
 size_t hv_lds_bytes(const NvOpDev &op, int prec) {
     const size_t el = prec == 0 ? sizeof(double) : sizeof(float);
     size_t off = ((size_t)op.ncodes * op.nmax * el + 15) & ~(size_t)15;
-    return off + (size_t)op.M * kBlock;
+    return off + 2 * (size_t)op.M * kBlock;  // digits of up to two rows per thread
 }
 
 int hv_blocks(const NvOpDev &op) { return (op.D + kBlock - 1) / kBlock; }
@@ -401,9 +525,22 @@ static hipError_t hv_full_ns(const NvOpDev &op, int prec, const double *x, doubl
     return hipGetLastError();
 }
 
+// Two rows per thread pays in FP32 on large operators (cfg3, 22M rows: 36.7 ->
+// 29.7 ms per H*v); in FP64 (55.6 -> 62.2 ms) and below a few million rows
+// (cfg2: 53.8 -> 57.4 us) the one-row kernel is faster.
+constexpr int kTwoRowMinRows = 1 << 22;
+
 template <int NS>
 static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     const size_t lds = hv_lds_bytes(op, prec);
+    if (prec != 0 && op.D >= kTwoRowMinRows) {
+        const unsigned g = (unsigned)((hv_blocks(op) + 1) / 2);
+        if (prec == 1)
+            hipLaunchKernelGGL((hv_iter2_kernel<NS, float, float>), dim3(g), dim3(kBlock), lds, s, op, st);
+        else
+            hipLaunchKernelGGL((hv_iter2_kernel<NS, __hip_bfloat16, float>), dim3(g), dim3(kBlock), lds, s, op, st);
+        return hipGetLastError();
+    }
     if (prec == 0)
         hipLaunchKernelGGL((hv_iter_kernel<NS, double, double>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
     else if (prec == 1)

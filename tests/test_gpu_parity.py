@@ -130,6 +130,22 @@ def test_lanczos_bf16_input_tracks_fp64():
     assert rel.max() > 1e-7  # it is not running the FP32 path
 
 
+def test_generic_two_row_kernel_on_a_large_operator():
+    """4,194,304 rows in FP32: the table-driven H*v runs two rows per thread
+    (large operators); its first coefficients agree with the FP64 oracle to
+    the FP32 level, as the one-row kernel does on cfg1."""
+    p = make_problem(2, [16, 16, 16, 16, 16, 4], 2, (400, 3000), 250.0, 60.0, 40.0, seed=13)
+    assert p.n // p.nstates == 1 << 22
+    ref = Oracle(p).lanczos(3)
+    with Engine(p) as eng:
+        eng.select_hv_kernel(True)
+        eng.lanczos_init(niter=3, prec="f32")
+        eng.run(1, 3)
+        a, b, _ = eng.coefficients()
+    np.testing.assert_allclose(a, ref["alpha"], rtol=5e-6)
+    np.testing.assert_allclose(b[1:3], ref["beta"][1:3], rtol=5e-5)
+
+
 def test_ritz_matches_givens():
     p = workload("cfg1")
     ref = Oracle(p).lanczos(300)
