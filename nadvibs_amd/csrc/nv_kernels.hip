@@ -212,16 +212,14 @@ __device__ __forceinline__ void offdiag_accumulate(const NvOpDev &op, const TIn 
     }
 }
 
-// The same for two rows of one thread (r0 and r0 + kBlock, digits at dig0 and
-// dig1): the step descriptors are read once for both, and the two rows'
+// The same for two rows of one thread (r0 and r1, digits at dig0 and dig1): the step descriptors are read once for both, and the two rows'
 // table and vector loads are independent, so they overlap.  A row that is
 // not live (past the end) has zero digits and is skipped.
 template <int NS, typename TIn, typename TAcc>
-__device__ __forceinline__ void offdiag_accumulate2(const NvOpDev &op, const TIn *__restrict__ x, int r0, bool l0,
-                                                    bool l1, const uint8_t *dig0, const uint8_t *dig1,
+__device__ __forceinline__ void offdiag_accumulate2(const NvOpDev &op, const TIn *__restrict__ x, int r0, int r1,
+                                                    bool l0, bool l1, const uint8_t *dig0, const uint8_t *dig1,
                                                     const TAcc *ftab, int tid, TAcc (&acc0)[NS], TAcc (&acc1)[NS]) {
     const TAcc *__restrict__ cm = cmat_of<TAcc>(op);
-    const int r1 = r0 + kBlock;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         acc0[s] = (TAcc)0;
@@ -375,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void hv_iter2_kernel(NvOpDev op, LzState st
     double part0 = 0.0, part1 = 0.0;
     if (l0) {
         TAcc acc0[NS], acc1[NS];
-        offdiag_accumulate2<NS, TIn, TAcc>(op, x, r0, l0, l1, dig, dig1, ftab, tid, acc0, acc1);
+        offdiag_accumulate2<NS, TIn, TAcc>(op, x, r0, r1, l0, l1, dig, dig1, ftab, tid, acc0, acc1);
         TAcc *y = reinterpret_cast<TAcc *>(st.y);
         double v[NS], dv[NS];
 #pragma unroll
@@ -1043,6 +1041,57 @@ __global__ __launch_bounds__(kBlock) void batch_hv_kernel(BatchDev b, int t0) {
     if (tid == 0) b.partials[t] = bs;
 }
 
+// batch_hv_kernel with two rows per thread: a workgroup of bs/2 threads
+// covers the tile's bs rows as two halves.
+template <int NS, typename TIn, typename TAcc>
+__global__ __launch_bounds__(kBlock) void batch_hv2_kernel(BatchDev b, int t0) {
+    const int t = t0 + blockIdx.x;
+    const BatchTile tl = b.tiles[t];
+    const NvOpDev op = b.ops[tl.prob];
+    uint8_t *dig;
+    const TAcc *ftab = stage_tables<TAcc>(op, dig);
+    uint8_t *dig1 = dig + op.M * kBlock;
+    const int tid = threadIdx.x;
+    const int half = b.bs / 2;
+    const int r0 = tl.row0 + tid, r1 = r0 + half;
+    const bool l0 = r0 < op.D, l1 = r1 < op.D;
+    const int64_t it = *b.iter;
+    const double *q = b.V + slot_of(it, 3) * b.Ntot + b.voff[tl.prob];
+    const TIn *x = b.vlo ? reinterpret_cast<const TIn *>(b.vlo) + b.loff[tl.prob] : reinterpret_cast<const TIn *>(q);
+    TAcc *y = reinterpret_cast<TAcc *>(b.y) + b.voff[tl.prob];
+    if (l0) digits_energy(op, r0, dig, tid);
+    else for (int k = 0; k < op.M; ++k) dig[k * kBlock + tid] = 0;
+    if (l1) digits_energy(op, r1, dig1, tid);
+    else for (int k = 0; k < op.M; ++k) dig1[k * kBlock + tid] = 0;
+    __syncthreads();
+    double part = 0.0;
+    if (l0) {
+        TAcc acc0[NS], acc1[NS];
+        offdiag_accumulate2<NS, TIn, TAcc>(op, x, r0, r1, l0, l1, dig, dig1, ftab, tid, acc0, acc1);
+        double v[NS], dv[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r0];
+        diag_apply<NS>(op, r0, dig, tid, v, dv);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            y[(size_t)s * op.D + r0] = acc0[s];
+            part += v[s] * (dv[s] + (double)acc0[s]);
+        }
+        if (l1) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r1];
+            diag_apply<NS>(op, r1, dig1, tid, v, dv);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                y[(size_t)s * op.D + r1] = acc1[s];
+                part += v[s] * (dv[s] + (double)acc1[s]);
+            }
+        }
+    }
+    const double bs = block_sum(part);
+    if (tid == 0) b.partials[t] = bs;
+}
+
 template <int NS, typename TAcc>
 __device__ __forceinline__ void batch_step_row(const NvOpDev &op, const double *q, const double *qm, double *res,
                                                const TAcc *y, double a, double bb, int64_t it, int r,
@@ -1112,8 +1161,11 @@ hipError_t launch_batch_normalize(const BatchDev &b, int prec, hipStream_t s) {
 
 template <int NS>
 static void batch_hv_ns(const BatchDev &b, int prec, size_t lds, int t0, int nt, hipStream_t s) {
+    // FP32: two rows per thread, as for large single operators (workgroups of at least one wave)
     if (prec == 0)
         hipLaunchKernelGGL((batch_hv_kernel<NS, double, double>), dim3(nt), dim3(b.bs), lds, s, b, t0);
+    else if (b.bs >= 128)
+        hipLaunchKernelGGL((batch_hv2_kernel<NS, float, float>), dim3(nt), dim3(b.bs / 2), lds, s, b, t0);
     else
         hipLaunchKernelGGL((batch_hv_kernel<NS, float, float>), dim3(nt), dim3(b.bs), lds, s, b, t0);
 }
