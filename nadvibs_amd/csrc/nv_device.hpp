@@ -19,11 +19,11 @@
 
 // The FP32/BF16 copy of a Lanczos vector that H*v reads is state-interleaved,
 // x_lo[r * nv_lo_stride(ns) + s], padded so that the states of one row are
-// one 4/8/16-byte load (padding lanes hold 0).
+// one 4/8/12/16-byte load (padding lanes hold 0; three states take 12 bytes).
 #ifdef __HIPCC__
 __host__ __device__
 #endif
-inline constexpr int nv_lo_stride(int ns) { return ns <= 1 ? 1 : ns <= 2 ? 2 : ns <= 4 ? 4 : ns <= 8 ? 8 : 16; }
+inline constexpr int nv_lo_stride(int ns) { return ns <= 3 ? ns : ns <= 4 ? 4 : ns <= 8 ? 8 : 16; }
 
 struct NvStep {
     int32_t stride;        // flat-index offset of the column, sum Delta_k * shft_k

@@ -111,7 +111,10 @@ __device__ __forceinline__ double nv_block_sum(double v) {
              "__device__ __forceinline__ nv_f2 nv_ld2(__amdgpu_buffer_rsrc_t r, unsigned o) { return "
              "__builtin_bit_cast(nv_f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o, 0, 0)); }\n"
              "__device__ __forceinline__ nv_f4 nv_ld4(__amdgpu_buffer_rsrc_t r, unsigned o) { return "
-             "__builtin_bit_cast(nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)o, 0, 0)); }\n";
+             "__builtin_bit_cast(nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)o, 0, 0)); }\n"
+             "typedef float nv_f3 __attribute__((ext_vector_type(3)));\n"
+             "__device__ __forceinline__ nv_f3 nv_ld3(__amdgpu_buffer_rsrc_t r, unsigned o) { return "
+             "__builtin_bit_cast(nv_f3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)o, 0, 0)); }\n";
     } else {
         s << "__device__ __forceinline__ T nv_ld(__amdgpu_buffer_rsrc_t r, unsigned o) { return "
              "__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o, 0, 0)); }\n";
@@ -151,6 +154,10 @@ __device__ __forceinline__ double nv_block_sum(double v) {
             o << " w0 = NV_FMA(h, nv_ld1(rx, o), w0);";
         } else if (nsp == 2) {
             o << " { const nv_f2 v = nv_ld2(rx, o);";
+            for (int q = 0; q < ns; ++q) o << " w" << q << " = NV_FMA(h, v[" << q << "], w" << q << ");";
+            o << " }";
+        } else if (nsp == 3) {
+            o << " { const nv_f3 v = nv_ld3(rx, o);";
             for (int q = 0; q < ns; ++q) o << " w" << q << " = NV_FMA(h, v[" << q << "], w" << q << ");";
             o << " }";
         } else {
