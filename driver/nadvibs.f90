@@ -27,12 +27,14 @@ program nadvibs_main
     integer(c_int64_t), allocatable, target :: nfunc(:), npirrep(:), istate(:)
     real(c_double), allocatable, target :: aomega(:), concoef(:), potterms(:, :, :), statew(:)
     real(c_double), allocatable, target :: bomega(:), dvec(:), tmat(:)
-    real(c_double), allocatable, target :: alpha(:), beta(:)
+    real(c_double), allocatable, target :: alpha(:), beta(:), omega(:, :)
     integer(c_int64_t), allocatable :: counts(:)
     type(nv_problem), target :: prob
     type(nv_lanczos_opts) :: opts
     type(c_ptr) :: ctx
     character(len=256) :: outdir, arg, env
+    character(kind=c_char, len=300), target :: proname
+    logical :: keep, have_pro
     integer :: nstblks, ntmax, i, j, k, iiter, nload, ios, prec
     integer(c_int64_t) :: it
     logical :: orthog
@@ -131,12 +133,20 @@ program nadvibs_main
         read(14, *); read(14, *) nload
         iiter = nload + 1
         niter = niter + iiter - 1
-        allocate(alpha(niter), beta(0:niter))
+        allocate(alpha(niter), beta(0:niter), omega(2, 0:niter))
+        omega = 0d0
         read(14, *); read(14, *) alpha(1:nload)
         read(14, *); read(14, *) beta(0:nload)
+        ! omega rows follow when the run re-orthogonalised (Source/top.f90:1032-1037)
+        if (orthog) then
+            read(14, *, iostat=ios); if (ios == 0) read(14, *, iostat=ios) omega(1, 0:nload)
+            if (ios == 0) read(14, *, iostat=ios); if (ios == 0) read(14, *, iostat=ios) omega(2, 0:nload)
+            if (ios /= 0) stop 'nadvibs: restart.log holds no omega rows for a re-orthogonalising run'
+        end if
         close(14)
     else
-        allocate(alpha(niter), beta(0:niter))
+        allocate(alpha(niter), beta(0:niter), omega(2, 0:niter))
+        omega = 0d0
     end if
 
     ! ---- Lanczos (Source/main.f90:63-71) ----
@@ -150,13 +160,25 @@ program nadvibs_main
     opts%checkpoint = 0
     opts%pro_tau = 0d0; opts%pro_eta = 0d0
     if (idroots > 0 .or. soroots > 0) opts%savevecs = 1
-    if (restart /= 0 .and. opts%savevecs /= 0) &
-        stop 'nadvibs: idroots/soroots need every Lanczos vector and cannot follow a restart'
+    ! every Lanczos vector is kept: the archive nadvibs.34 (Source/secondary.f90:273-308) and,
+    ! with re-orthogonalisation, its state (nadvibs.36) are written at the end and reloaded on restart
+    keep = orthog .or. opts%savevecs /= 0
+    proname = trim(outdir)//'nadvibs.36.0'//c_null_char
     call nv_check(nv_lanczos_init(ctx, opts), ctx, 'nv_lanczos_init')
     if (restart /= 0) then
         it = nload
-        call nv_check(nv_load_vectors(ctx, trim(outdir)//'nadvibs.35.0'//c_null_char, it, alpha, beta, c_null_ptr), &
-                      ctx, 'nv_load_vectors')
+        if (keep) then
+            inquire(file=trim(outdir)//'nadvibs.36.0', exist=have_pro)
+            if (orthog .and. have_pro) then
+                call nv_check(nv_load_archive(ctx, trim(outdir)//'nadvibs.34.0'//c_null_char, c_loc(proname), it), &
+                              ctx, 'nv_load_archive')
+            else
+                call nv_check(nv_load_archive(ctx, trim(outdir)//'nadvibs.34.0'//c_null_char, c_null_ptr, it), &
+                              ctx, 'nv_load_archive')
+            end if
+        end if
+        call nv_check(nv_load_vectors(ctx, trim(outdir)//'nadvibs.35.0'//c_null_char, it, alpha, beta, &
+                                      c_loc(omega)), ctx, 'nv_load_vectors')
     end if
     call get_environment_variable('NV_LOOP', env)
     if (trim(env) == '1') then
@@ -172,7 +194,11 @@ program nadvibs_main
     else
         call nv_check(nv_lanczos_run(ctx, int(iiter, c_int64_t), int(niter, c_int64_t)), ctx, 'nv_lanczos_run')
     end if
-    call nv_check(nv_get_coefficients(ctx, c_loc(alpha), c_loc(beta), c_null_ptr), ctx, 'nv_get_coefficients')
+    if (orthog) then
+        call nv_check(nv_get_coefficients(ctx, c_loc(alpha), c_loc(beta), c_loc(omega)), ctx, 'nv_get_coefficients')
+    else
+        call nv_check(nv_get_coefficients(ctx, c_loc(alpha), c_loc(beta), c_null_ptr), ctx, 'nv_get_coefficients')
+    end if
     call cpu_time(t1)
     write(13, '(a)') '  - Lanczos Iterations Completed - '
     write(13, '(a,f14.5,a)') '  Time Required: ', t1 - t0, ' secs.'
@@ -187,8 +213,21 @@ program nadvibs_main
     write(15, *) alpha(1:niter)
     write(15, '(a)') ' BETA COEFFICIENTS -- off-diagonal T matrix elements'
     write(15, *) beta(0:niter)
+    if (orthog) then
+        write(15, '(a)') ' OMEGA(1) -- ORTHOGONALITY INFORMATION'
+        write(15, *) omega(1, 0:niter)
+        write(15, '(a)') ' OMEGA(2) -- ORTHOGONALITY INFORMATION'
+        write(15, *) omega(2, 0:niter)
+    end if
     close(15)
     call nv_check(nv_save_vectors(ctx, trim(outdir)//'nadvibs.35.0'//c_null_char), ctx, 'nv_save_vectors')
+    if (orthog) then
+        call nv_check(nv_save_archive(ctx, trim(outdir)//'nadvibs.34.0'//c_null_char, c_loc(proname)), &
+                      ctx, 'nv_save_archive')
+    else if (keep) then
+        call nv_check(nv_save_archive(ctx, trim(outdir)//'nadvibs.34.0'//c_null_char, c_null_ptr), &
+                      ctx, 'nv_save_archive')
+    end if
 
     call compute_eigenvalues()
     close(13)

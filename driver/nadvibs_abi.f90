@@ -125,6 +125,20 @@ module nadvibs_abi
             real(c_double), intent(in) :: alpha(*), beta(*)
             type(c_ptr), value :: omega
         end function
+        ! pro_path: c_loc of a NUL-terminated name, or c_null_ptr
+        integer(c_int32_t) function nv_save_archive(ctx, path, pro_path) bind(C, name="nv_save_archive")
+            import :: c_int32_t, c_ptr, c_char
+            type(c_ptr), value :: ctx
+            character(kind=c_char), intent(in) :: path(*)
+            type(c_ptr), value :: pro_path
+        end function
+        integer(c_int32_t) function nv_load_archive(ctx, path, pro_path, iter) bind(C, name="nv_load_archive")
+            import :: c_int32_t, c_ptr, c_char, c_int64_t
+            type(c_ptr), value :: ctx
+            character(kind=c_char), intent(in) :: path(*)
+            type(c_ptr), value :: pro_path
+            integer(c_int64_t), value :: iter
+        end function
     end interface
 
 contains

@@ -325,6 +325,20 @@ int32_t nv_save_vectors(nv_ctx *ctx, const char *path);
 int32_t nv_load_vectors(nv_ctx *ctx, const char *path, int64_t iter,
                         const double *alpha, const double *beta,
                         const double *omega);
+/* A run that keeps every Lanczos vector (reorthog > 0 or savevecs) restarts
+ * from its archive, as the reference reopens nadvibs.34.<rank>
+ * (Source/top.f90:569-572).  nv_save_archive writes q_1..q_n (n = last
+ * iteration) as n records of nstates*dimen doubles (that file's layout for
+ * one process) and, when pro_path is not NULL, the state of partial
+ * re-orthogonalisation (omega rows, pending ranges, noise position) in the
+ * library's own format, so that the restarted run continues exactly.
+ * nv_load_archive restores them for iteration iter after nv_lanczos_init and
+ * before nv_load_vectors; without pro_path the re-orthogonalisation starts
+ * afresh from the omega passed to nv_load_vectors, as in the reference. */
+int32_t nv_save_archive(nv_ctx *ctx, const char *archive_path,
+                        const char *pro_path);
+int32_t nv_load_archive(nv_ctx *ctx, const char *archive_path,
+                        const char *pro_path, int64_t iter);
 
 /* ---- measurement hooks (no counterpart in the reference: they serve the
  *      benchmark's live roofline) ------------------------------------------ */

@@ -93,6 +93,8 @@ SIGNATURES = [
                                      ctypes.c_double, _dp]),
     ("nv_save_vectors", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p]),
     ("nv_load_vectors", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64, _dp, _dp, _dp]),
+    ("nv_save_archive", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]),
+    ("nv_load_archive", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64]),
     ("nv_profile_enable", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     ("nv_profile_read", ctypes.c_int32, [ctypes.c_void_p, _dp, _i64p]),
     ("nv_hv_kernel_info", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]),

@@ -36,6 +36,8 @@ struct nv_ctx {
     int64_t *d_iter = nullptr;
     double *d_startvals = nullptr;
     int64_t last_iter = 0;       // last iteration whose lanczos_step was issued
+    int64_t archive_iter = 0;    // iteration up to which nv_load_archive restored the kept basis
+    bool pro_loaded = false;     // nv_load_archive restored the re-orthogonalisation state
     int64_t next_update = 1;     // iteration expected by update_lanczos
     std::vector<void *> lz_allocs;
     nv::ProState pro{};
@@ -463,6 +465,8 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     c->lz_ready = true;
     c->last_iter = 0;
     c->next_update = 1;
+    c->archive_iter = 0;
+    c->pro_loaded = false;
     return NV_OK;
 }
 
@@ -1339,9 +1343,10 @@ int32_t nv_load_vectors(nv_ctx *c, const char *path, int64_t iter, const double 
     if (!c || !path || !alpha || !beta) return NV_ERR_ARG;
     if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
     if (iter < 1 || iter >= c->opts.niter) return fail(c, NV_ERR_ARG, "restart iteration out of range");
-    if (c->opts.reorthog > 0 || c->opts.savevecs != 0 || c->cp)
-        return fail(c, NV_ERR_STATE, "a restart cannot rebuild the archive or checkpoints of earlier Lanczos vectors");
-    (void)omega;
+    if (c->cp) return fail(c, NV_ERR_STATE, "a restart cannot rebuild the checkpoints of earlier Lanczos vectors");
+    if ((c->opts.reorthog > 0 || c->opts.savevecs != 0) && c->archive_iter != iter)
+        return fail(c, NV_ERR_STATE, "a run that keeps every Lanczos vector restarts from its archive: "
+                                     "call nv_load_archive for the same iteration first");
     const int64_t N = c->op.N;
     std::vector<double> buf(3 * N);
     FILE *f = std::fopen(path, "rb");
@@ -1358,8 +1363,118 @@ int32_t nv_load_vectors(nv_ctx *c, const char *path, int64_t iter, const double 
     }
     NV_HIP(c, hipMemcpy(c->st.alpha + 1, alpha, iter * sizeof(double), hipMemcpyHostToDevice));
     NV_HIP(c, hipMemcpy(c->st.beta, beta, (iter + 1) * sizeof(double), hipMemcpyHostToDevice));
+    // the reference's restart.log carries omega (Source/top.f90:564-568); a PRO state
+    // loaded by nv_load_archive already holds it
+    if (omega && c->opts.reorthog > 0 && !c->pro_loaded) {
+        hipError_t e = nv::pro_put_omega(c->pro, omega, iter);
+        if (e != hipSuccess) return hip_fail(c, e, "omega restore");
+    }
     c->last_iter = iter;
     c->next_update = iter + 1;
+    return NV_OK;
+}
+
+static __global__ void to_f32_kernel(const double *__restrict__ x, float *__restrict__ y, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (float)x[i];
+}
+
+static const char kProMagic[8] = {'N', 'V', 'P', 'R', 'O', '0', '0', '1'};
+
+int32_t nv_save_archive(nv_ctx *c, const char *archive_path, const char *pro_path) {
+    if (!c || !archive_path) return NV_ERR_ARG;
+    if (!c->lz_ready || c->last_iter < 1) return fail(c, NV_ERR_STATE, "no completed iteration");
+    if (c->opts.reorthog <= 0 && c->opts.savevecs == 0)
+        return fail(c, NV_ERR_STATE, "the archive needs a run that keeps every Lanczos vector (savevecs or reorthog)");
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t N = c->op.N, n = c->last_iter;
+    FILE *f = std::fopen(archive_path, "wb");
+    if (!f) return fail(c, NV_ERR_IO, std::string("cannot open ") + archive_path);
+    std::vector<double> buf(N);
+    for (int64_t j = 1; j <= n; ++j) {
+        hipError_t e = hipMemcpy(buf.data(), c->st.V + (j % c->st.nslots) * N, N * sizeof(double),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            std::fclose(f);
+            return hip_fail(c, e, "archive copy");
+        }
+        if (std::fwrite(buf.data(), sizeof(double), N, f) != (size_t)N) {
+            std::fclose(f);
+            return fail(c, NV_ERR_IO, std::string("short write to ") + archive_path);
+        }
+    }
+    std::fclose(f);
+    if (!pro_path) return NV_OK;
+    if (c->opts.reorthog <= 0) return fail(c, NV_ERR_STATE, "no re-orthogonalisation state in this run");
+    std::vector<int64_t> ints;
+    std::vector<double> reals;
+    hipError_t e = nv::pro_export(c->pro, n, ints, reals);
+    if (e != hipSuccess) return hip_fail(c, e, "PRO state");
+    f = std::fopen(pro_path, "wb");
+    if (!f) return fail(c, NV_ERR_IO, std::string("cannot open ") + pro_path);
+    const int64_t sizes[2] = {(int64_t)ints.size(), (int64_t)reals.size()};
+    bool ok = std::fwrite(kProMagic, 1, 8, f) == 8 && std::fwrite(sizes, sizeof(int64_t), 2, f) == 2 &&
+              std::fwrite(ints.data(), sizeof(int64_t), ints.size(), f) == ints.size() &&
+              std::fwrite(reals.data(), sizeof(double), reals.size(), f) == reals.size();
+    std::fclose(f);
+    if (!ok) return fail(c, NV_ERR_IO, std::string("short write to ") + pro_path);
+    return NV_OK;
+}
+
+int32_t nv_load_archive(nv_ctx *c, const char *archive_path, const char *pro_path, int64_t iter) {
+    if (!c || !archive_path) return NV_ERR_ARG;
+    if (!c->lz_ready) return fail(c, NV_ERR_STATE, "nv_lanczos_init has not been called");
+    if (c->opts.reorthog <= 0 && c->opts.savevecs == 0)
+        return fail(c, NV_ERR_STATE, "the archive needs a run that keeps every Lanczos vector (savevecs or reorthog)");
+    if (iter < 1 || iter >= c->opts.niter) return fail(c, NV_ERR_ARG, "restart iteration out of range");
+    if (pro_path && c->opts.reorthog <= 0) return fail(c, NV_ERR_STATE, "no re-orthogonalisation in this run");
+    NV_HIP(c, hipSetDevice(c->device));
+    NV_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t N = c->op.N;
+    FILE *f = std::fopen(archive_path, "rb");
+    if (!f) return fail(c, NV_ERR_IO, std::string("cannot open ") + archive_path);
+    std::vector<double> buf(N);
+    for (int64_t j = 1; j <= iter; ++j) {
+        if (std::fread(buf.data(), sizeof(double), N, f) != (size_t)N) {
+            std::fclose(f);
+            return fail(c, NV_ERR_IO, std::string("short read from ") + archive_path);
+        }
+        double *dst = c->st.V + (j % c->st.nslots) * N;
+        hipError_t e = hipMemcpy(dst, buf.data(), N * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess && c->st.Vf) {
+            hipLaunchKernelGGL(to_f32_kernel, dim3(1024), dim3(256), 0, c->stream, dst,
+                               c->st.Vf + (j % c->st.nslots) * c->st.ldf, N);
+            e = hipStreamSynchronize(c->stream);
+        }
+        if (e != hipSuccess) {
+            std::fclose(f);
+            return hip_fail(c, e, "archive load");
+        }
+    }
+    std::fclose(f);
+    c->pro_loaded = false;
+    if (pro_path) {
+        f = std::fopen(pro_path, "rb");
+        if (!f) return fail(c, NV_ERR_IO, std::string("cannot open ") + pro_path);
+        char magic[8];
+        int64_t sizes[2] = {0, 0};
+        bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kProMagic, 8) == 0 &&
+                  std::fread(sizes, sizeof(int64_t), 2, f) == 2 && sizes[0] >= 0 && sizes[1] >= 0 &&
+                  sizes[0] < (int64_t)1 << 32 && sizes[1] < (int64_t)1 << 32;
+        std::vector<int64_t> ints(ok ? sizes[0] : 0);
+        std::vector<double> reals(ok ? sizes[1] : 0);
+        ok = ok && std::fread(ints.data(), sizeof(int64_t), ints.size(), f) == ints.size() &&
+             std::fread(reals.data(), sizeof(double), reals.size(), f) == reals.size();
+        std::fclose(f);
+        if (!ok) return fail(c, NV_ERR_IO, std::string("not a re-orthogonalisation state file: ") + pro_path);
+        hipError_t e = nv::pro_import(c->pro, iter, ints, reals);
+        if (e == hipErrorInvalidValue)
+            return fail(c, NV_ERR_ARG, "the re-orthogonalisation state is of another iteration or does not fit this run");
+        if (e != hipSuccess) return hip_fail(c, e, "PRO state load");
+        c->pro_loaded = true;
+    }
+    c->archive_iter = iter;
     return NV_OK;
 }
 

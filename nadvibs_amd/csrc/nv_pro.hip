@@ -627,6 +627,69 @@ hipError_t pro_partial_orthog(ProState &ps, const NvOpDev &dev, const LzState &s
     return hipGetLastError();
 }
 
+// ints: [0] n, [1] cap, [2] jcap, [3] njl, [4..11] meta, then lo[cap], ro[cap], jlist[jcap];
+// reals: omega(1, 0..n+1), omega(2, 0..n+1).
+hipError_t pro_export(const ProState &ps, int64_t n, std::vector<int64_t> &ints, std::vector<double> &reals) {
+    const ProDev &p = ps.d;
+    if (n + 1 >= p.ostride) return hipErrorInvalidValue;
+    ints.assign(12 + 2 * p.cap + p.jcap, 0);
+    ints[0] = n;
+    ints[1] = p.cap;
+    ints[2] = p.jcap;
+    int32_t njl = 0;
+    hipError_t e = hipMemcpy(&njl, p.njl, sizeof(int32_t), hipMemcpyDeviceToHost);
+    ints[3] = njl;
+    if (e == hipSuccess) e = hipMemcpy(ints.data() + 4, p.meta, 8 * sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ints.data() + 12, p.lo, p.cap * sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(ints.data() + 12 + p.cap, p.ro, p.cap * sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(ints.data() + 12 + 2 * p.cap, p.jlist, p.jcap * sizeof(int64_t), hipMemcpyDeviceToHost);
+    reals.assign(2 * (n + 2), 0.0);
+    if (e == hipSuccess) e = hipMemcpy(reals.data(), p.omega, (n + 2) * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(reals.data() + n + 2, p.omega + p.ostride, (n + 2) * sizeof(double), hipMemcpyDeviceToHost);
+    return e;
+}
+
+hipError_t pro_import(ProState &ps, int64_t n, const std::vector<int64_t> &ints, const std::vector<double> &reals) {
+    ProDev &p = ps.d;
+    if (ints.size() < 12 || ints[0] != n || n + 1 >= p.ostride) return hipErrorInvalidValue;
+    const int64_t cap = ints[1], jcap = ints[2];
+    if (cap < 0 || jcap < 0 || (int64_t)ints.size() != 12 + 2 * cap + jcap || (int64_t)reals.size() != 2 * (n + 2))
+        return hipErrorInvalidValue;
+    const int32_t njl = (int32_t)ints[3];
+    // ranges and the list past this run's capacity cannot be restored
+    if (ints[5] > p.cap || njl > p.jcap) return hipErrorInvalidValue;
+    const int64_t ncap = std::min(cap, p.cap), njc = std::min(jcap, p.jcap);
+    hipError_t e = hipMemcpy(p.njl, &njl, sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p.meta, ints.data() + 4, 8 * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(p.lo, 0, p.cap * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemset(p.ro, 0, p.cap * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemcpy(p.lo, ints.data() + 12, ncap * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p.ro, ints.data() + 12 + cap, ncap * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p.jlist, ints.data() + 12 + 2 * cap, njc * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(p.omega, 0, 2 * p.ostride * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(p.omega, reals.data(), (n + 2) * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p.omega + p.ostride, reals.data() + n + 2, (n + 2) * sizeof(double), hipMemcpyHostToDevice);
+    return e;
+}
+
+hipError_t pro_put_omega(ProState &ps, const double *omega, int64_t n) {
+    ProDev &p = ps.d;
+    if (n + 1 >= p.ostride) return hipErrorInvalidValue;
+    std::vector<double> h(2 * p.ostride, 0.0);
+    hipError_t e = hipMemcpy(h.data(), p.omega, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (int64_t j = 0; j <= n; ++j) {
+        h[j] = omega[2 * j];
+        h[p.ostride + j] = omega[2 * j + 1];
+    }
+    return hipMemcpy(p.omega, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+}
+
 hipError_t pro_get_omega(const ProState &ps, double *omega, int64_t n) {
     const ProDev &p = ps.d;
     std::vector<double> h(2 * p.ostride);

@@ -74,6 +74,14 @@ int64_t pro_check_period(const ProState &p);
 hipError_t pro_set_omega(ProState &p, const NvOpDev &dev, const LzState &st, int check, hipStream_t s);
 hipError_t pro_partial_orthog(ProState &p, const NvOpDev &dev, const LzState &st, hipStream_t s);
 hipError_t pro_get_omega(const ProState &p, double *omega, int64_t n);
+// The state of partial re-orthogonalisation after iteration n as a flat
+// record (omega rows 0..n+1, ranges, list, counters, noise position), and its
+// restoration into a run initialised for at least n + 1 iterations.
+hipError_t pro_export(const ProState &p, int64_t n, std::vector<int64_t> &ints, std::vector<double> &reals);
+// Returns hipErrorInvalidValue when the record does not fit this run.
+hipError_t pro_import(ProState &p, int64_t n, const std::vector<int64_t> &ints, const std::vector<double> &reals);
+// omega rows 0..n from the host (omega[2j] = omega(1,j), omega[2j+1] = omega(2,j)).
+hipError_t pro_put_omega(ProState &p, const double *omega, int64_t n);
 void pro_stats(const ProState &p, int64_t *events, int64_t *touched);
 
 // Franck-Condon start vector into dst (n doubles), normalised.

@@ -289,10 +289,24 @@ class Engine:
     def save_vectors(self, path):
         self._check(self.lib.nv_save_vectors(self.h, os.fsencode(path)))
 
-    def load_vectors(self, path, it, alpha, beta):
+    def load_vectors(self, path, it, alpha, beta, omega=None):
+        """omega: [2, it+1] rows omega(1, 0:it), omega(2, 0:it) of restart.log, or None."""
         alpha = np.ascontiguousarray(alpha, dtype=np.float64)
         beta = np.ascontiguousarray(beta, dtype=np.float64)
-        self._check(self.lib.nv_load_vectors(self.h, os.fsencode(path), it, dptr(alpha), dptr(beta), None))
+        om = None
+        if omega is not None:
+            om = np.ascontiguousarray(np.asarray(omega, dtype=np.float64)[:, :it + 1].T)  # [j][row]
+        self._check(self.lib.nv_load_vectors(self.h, os.fsencode(path), it, dptr(alpha), dptr(beta),
+                                             None if om is None else dptr(om)))
+
+    def save_archive(self, path, pro_path=None):
+        """The kept basis q_1..q_n (and the re-orthogonalisation state)."""
+        self._check(self.lib.nv_save_archive(self.h, os.fsencode(path),
+                                             None if pro_path is None else os.fsencode(pro_path)))
+
+    def load_archive(self, path, it, pro_path=None):
+        self._check(self.lib.nv_load_archive(self.h, os.fsencode(path),
+                                             None if pro_path is None else os.fsencode(pro_path), it))
 
 
 def select_roots(evals, inten, bji, bconv=0):

@@ -135,3 +135,109 @@ def test_engine_resumes_after_load_vectors(tmp_path):
         a2, b2, _ = eng.coefficients()
     np.testing.assert_allclose(a2, a1, rtol=1e-12)
     np.testing.assert_allclose(b2, b1, rtol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_engine_pro_restart_from_archive_is_exact(tmp_path, prec):
+    """Partial re-orthogonalisation (noise on, checks every 30): 150 iterations,
+    archive + re-orthogonalisation state + restart vectors saved, a new
+    context restores them and runs 151..300: the coefficients and the PRO
+    counters are those of one uninterrupted 300-iteration run."""
+    p = workload("cfg1")
+    kw = dict(niter=300, reorthog=1, chkorthog=30, prec=prec, seed=7)
+    with Engine(p) as eng:
+        eng.lanczos_init(**kw)
+        eng.run(1, 300)
+        a1, b1, _ = eng.coefficients()
+        st1 = eng.pro_stats()
+    with Engine(p) as eng:
+        eng.lanczos_init(**kw)
+        eng.run(1, 150)
+        a, b, om = eng.coefficients(150, omega=True)
+        eng.save_vectors(tmp_path / "v35")
+        eng.save_archive(tmp_path / "v34", tmp_path / "v36")
+    with Engine(p) as eng:
+        eng.lanczos_init(**kw)
+        eng.load_archive(tmp_path / "v34", 150, tmp_path / "v36")
+        eng.load_vectors(tmp_path / "v35", 150, a, b, om)
+        eng.run(151, 300)
+        a2, b2, _ = eng.coefficients()
+        st2 = eng.pro_stats()
+    assert st2 == st1
+    np.testing.assert_allclose(a2, a1, rtol=1e-12)
+    np.testing.assert_allclose(b2, b1, rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_engine_pro_restart_reference_style_keeps_orthogonality(tmp_path):
+    """Without the re-orthogonalisation state (the reference restarts PRO
+    afresh from the omega rows of restart.log): the continued basis stays
+    orthogonal and the converged roots match the uninterrupted run's."""
+    p = workload("cfg1")
+    kw = dict(niter=300, reorthog=1, chkorthog=30, seed=7)
+    with Engine(p) as eng:
+        eng.lanczos_init(**kw)
+        eng.run(1, 300)
+        a1, b1, _ = eng.coefficients()
+        ev1, _, bj1, _ = eng.ritz(a1, b1)
+    with Engine(p) as eng:
+        eng.lanczos_init(**kw)
+        eng.run(1, 150)
+        a, b, om = eng.coefficients(150, omega=True)
+        eng.save_vectors(tmp_path / "v35")
+        eng.save_archive(tmp_path / "v34")
+    with Engine(p) as eng:
+        eng.lanczos_init(**kw)
+        eng.load_archive(tmp_path / "v34", 150)
+        eng.load_vectors(tmp_path / "v35", 150, a, b, om)
+        eng.run(151, 300)
+        a2, b2, _ = eng.coefficients()
+        ev2, _, bj2, _ = eng.ritz(a2, b2)
+        Q = np.array([eng.lanczos_vector(j) for j in range(1, 301)])
+    np.testing.assert_allclose(a2[:150], a1[:150], rtol=1e-12)
+    assert np.max(np.abs(Q @ Q.T - np.eye(300))) < 1e-6
+    conv = ev1[bj1 < 1e-9]
+    assert conv.size > 20
+    for e in conv:
+        assert np.min(np.abs(ev2 - e)) * 219474.6313705 < 1e-6
+
+
+@pytest.mark.gpu
+def test_driver_restart_with_reorthogonalisation(tmp_path):
+    """reorthog=1: 150 iterations, then restart=1 with 150 more.  restart.log
+    carries the omega rows (Source/top.f90:1032-1037), the archive nadvibs.34.0
+    and the PRO state nadvibs.36.0 are written, and the converged roots of
+    the 300-iteration table match a single 300-iteration run within 1e-6
+    cm^-1 (the reference's range arrays scale with niter, so the two runs
+    may take different re-orthogonalisation decisions)."""
+    _build()
+    p = workload("cfg1")
+    p.reorthog = 1
+    p.chkorthog = 30
+    full = tmp_path / "full"
+    part = tmp_path / "part"
+    p.niter = 300
+    write_problem(full, p)
+    run_driver(full)
+    p.niter = 150
+    write_problem(part, p)
+    run_driver(part)
+    log = open(part / "restart.log").read()
+    assert "OMEGA(1) -- ORTHOGONALITY INFORMATION" in log and "OMEGA(2)" in log
+    assert os.path.getsize(part / "nadvibs.34.0") == 150 * p.n * 8
+    assert os.path.exists(part / "nadvibs.36.0")
+    p.restart = 1
+    write_problem(part, p)
+    run_driver(part)
+    _, n1, a1, b1, _ = read_restart_log(full / "restart.log")
+    _, n2, a2, b2, _ = read_restart_log(part / "restart.log")
+    assert n1 == n2 == 300
+    np.testing.assert_allclose(a2[:150], a1[:150], rtol=1e-12)
+    t1 = parse_table(full / "output.dat")
+    t2 = parse_table(part / "output.dat")
+    c1 = t1[t1[:, 0] < 1e-9, 1]
+    c2 = t2[t2[:, 0] < 1e-9, 1]
+    assert c1.size > 20
+    for e in c1:
+        assert np.min(np.abs(c2 - e)) < 1e-6
