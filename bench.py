@@ -256,6 +256,41 @@ def cfg3_probe(sync, iters=8):
     return out
 
 
+def cfg3_full(sync, niter=3000):
+    """The whole cfg3 job (3000 iterations, reorthog 0) in FP32 and FP64 H*v,
+    with the converged roots of the lowest 3000 cm^-1 compared (about 5
+    minutes; not part of the default run)."""
+    from nadvibs_amd.engine import Engine
+    from nadvibs_amd.synth import workload
+    p = workload("cfg3")
+    runs = {}
+    with Engine(p, device=0) as eng:
+        for prec in ("f32", "f64"):
+            eng.lanczos_init(niter=niter, reorthog=0, prec=prec)
+            sync()
+            t0 = time.perf_counter()
+            for i0 in range(1, niter + 1, 500):
+                i1 = min(niter, i0 + 499)
+                eng.run(i0, i1)
+                sync()
+                print(f"# cfg3 {prec}: {i1} iterations at {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                      flush=True)
+            a, b, _ = eng.coefficients()
+            dt = time.perf_counter() - t0
+            runs[prec] = (dt,) + tuple(eng.ritz(a, b)[:3])
+    dt32, ev32, it32, bj32 = runs["f32"]
+    dt64, ev64, it64, bj64 = runs["f64"]
+    low = (ev64 - ev64[0]) * AU2WAVE < 3000.0
+    keep = (bj64 < 1e-9) & low
+    dE, _, nconv = compare(np.where(keep, ev64, np.inf), it64, np.where(keep, bj64, 1.0), ev32, it32, bj32)
+    return {"workload": f"cfg3: nstates={p.nstates} nmodes={p.nmodes} basis={p.nfunc} ordr={p.ordr} N={p.n}",
+            "niter": niter, "reorthog": 0, "f32_s": round(dt32, 2), "f64_s": round(dt64, 2),
+            "f32_iter_per_s": round(niter / dt32, 2), "f64_iter_per_s": round(niter / dt64, 2),
+            "speedup_f32_vs_f64": round(dt64 / dt32, 3), "n_converged_below_3000_cm1": nconv,
+            "max_abs_dE_cm1": float(f"{dE:.3e}"),
+            "note": "without re-orthogonalisation converged roots have ghost copies, so intensities are not compared"}
+
+
 def load_traffic(path):
     """HBM bytes per H*v launch measured with rocprofv3 counters, if a
     summary with a 'traffic_bytes_per_launch:' line exists."""
@@ -306,6 +341,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-problem batched workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the spectrum-gradient workload")
     ap.add_argument("--no-cfg3", action="store_true", help="skip the 88.5M-row ordr-3 probe")
+    ap.add_argument("--cfg3-full", action="store_true",
+                    help="only the whole 3000-iteration cfg3 job in both precisions (about 5 minutes)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only the calibration copy and H*v launches, for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -316,6 +353,10 @@ def main():
 
     if args.pmc_probe:
         pmc_probe(args)
+        return
+    if args.cfg3_full:
+        import torch
+        print(json.dumps(cfg3_full(torch.cuda.synchronize)), flush=True)
         return
 
     import torch
