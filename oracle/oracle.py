@@ -230,3 +230,11 @@ def identify_roots(alpha, beta, basis, nroots, idroots, nstates):
     out["percent"] = 100.0 * np.abs(w / w.sum(axis=1, keepdims=True))
     out["overlap"] = np.einsum("ikr,jlr->ijkl", Y3, Y3)
     return out
+
+
+def sticks_envelope(x, h, grid, sigma):
+    """Sum of Gaussians h_j exp(-(g - x_j)^2 / (2 sigma^2)) on grid: the
+    aperture of convolve.py:28-48 (before its division by the maximum) and of
+    the reference's convolve/sub.f90 (sigma = fwhm / (2 sqrt(2 ln 2)))."""
+    x, h, grid = (np.asarray(v, dtype=np.float64) for v in (x, h, grid))
+    return (h[None, :] * np.exp(-(grid[:, None] - x[None, :]) ** 2 / (2.0 * sigma * sigma))).sum(axis=1)
