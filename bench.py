@@ -73,9 +73,9 @@ def compare(ev64, it64, bj64, ev, it, bj, tol_conv=1e-9):
     return (max(dE) if dE else float("nan")), (max(dI) if dI else float("nan")), len(conv)
 
 
-def cpu_baseline(prob, seconds):
+def cpu_baseline(prob, seconds, seed=12345):
     """The oracle (scalar FP64 restatement) on this host's CPU, 1 thread, on
-    the first iterations of the same job."""
+    the first iterations of the same job; also returns its coefficients."""
     from oracle.oracle import Oracle
     orc = Oracle(prob)
     t0 = time.perf_counter()
@@ -83,10 +83,12 @@ def cpu_baseline(prob, seconds):
     t_hv = time.perf_counter() - t0
     k = max(2, int(seconds / max(t_hv * 1.1, 1e-6)))
     t0 = time.perf_counter()
-    orc.lanczos(k, reorthog=prob.reorthog, chkorthog=prob.chkorthog)
+    r = orc.lanczos(k, reorthog=prob.reorthog, chkorthog=prob.chkorthog, seed=seed)
     dt = time.perf_counter() - t0
     return dict(value=k / dt, unit="Lanczos iter/s", cores=1, kind="port",
-                sample=f"first {k} Lanczos iterations of the same workload (FP64 oracle, 1 thread), {dt:.1f} s")
+                sample=f"first {k} Lanczos iterations of the same workload (FP64 oracle, 1 thread), {dt:.1f} s"), r
+
+
 
 
 def batched_cfg5(sync, cpu_seconds):
@@ -449,7 +451,15 @@ def main():
     if not args.no_cfg3:
         rec["cfg3"] = cfg3_probe(sync)
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
+        rec["cpu_baseline"], ref = cpu_baseline(prob, args.cpu_seconds, args.seed)
+        # the GPU jobs' first coefficients against the FP64 CPU run of the same iterations
+        k = ref["alpha"].size
+        check = {"iterations": k,
+                 "alpha_max_rel_diff_f32": float(f"{np.max(np.abs(out32[0][:k] - ref['alpha']) / np.abs(ref['alpha'])):.3e}")}
+        if res64 is not None:
+            check["alpha_max_rel_diff_f64"] = float(
+                f"{np.max(np.abs(res64[3][0][:k] - ref['alpha']) / np.abs(ref['alpha'])):.3e}")
+        rec["vs_cpu_fp64"] = check
     print(json.dumps(rec), flush=True)
 
 
