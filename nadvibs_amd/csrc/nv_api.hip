@@ -49,6 +49,7 @@ struct nv_ctx {
 
     // operator-specialised H*v kernels per precision (F64, F32)
     bool jit_on = true;
+    const NvTermShape *tshape = nullptr;  // the shaped-kernel descriptors (dev.tshape when in use)
     nv::JitKernels jit[2];
     std::string jit_log;
     float *hx32 = nullptr;
@@ -150,7 +151,7 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
     c->ztoler = prob->ztoler;
     {
         const char *k = std::getenv("NV_HV_KERNEL");
-        c->jit_on = !(k && std::strcmp(k, "generic") == 0);
+        c->jit_on = !(k && (std::strcmp(k, "generic") == 0 || std::strcmp(k, "steps") == 0));
     }
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -172,6 +173,13 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
     if (e == hipSuccess) e = upload(c, op.dconst, &d.dconst, c->allocs);
     if (e == hipSuccess) e = upload(c, op.omega, &d.omega, c->allocs);
     if (e == hipSuccess) e = upload(c, nf32, &d.nfunc, c->allocs);
+    if (e == hipSuccess && op.shaped) e = upload(c, op.dev_shapes, &d.tshape, c->allocs);
+    for (int q = 0; q <= NV_NSHAPES; ++q) d.tgrp[q] = d.tshape ? op.shape_grp[q] : 0;
+    c->tshape = d.tshape;
+    {
+        const char *k = std::getenv("NV_HV_KERNEL");
+        if (k && std::strcmp(k, "steps") == 0) d.tshape = nullptr;
+    }
     if (e != hipSuccess) {
         g_create_error = std::string("device setup failed: ") + hipGetErrorString(e);
         free_list(c->allocs);
@@ -317,8 +325,11 @@ int32_t nv_hv_device(nv_ctx *c, int32_t prec, const double *x, double *y, void *
 int32_t nv_select_hv_kernel(nv_ctx *c, int32_t generic) {
     if (!c) return NV_ERR_ARG;
     const bool on = generic == 0;
-    if (on != c->jit_on) release_graphs(c);  // captured iterations name the kernel
+    const NvTermShape *ts = generic == 2 ? nullptr : c->tshape;
+    // captured iterations name the kernel
+    if (on != c->jit_on || ts != c->dev.tshape) release_graphs(c);
     c->jit_on = on;
+    c->dev.tshape = ts;
     return NV_OK;
 }
 
@@ -513,7 +524,9 @@ int32_t nv_hv_kernel_info(nv_ctx *c, int32_t prec, char *buf, int64_t len) {
     if (!c || !buf || len < 1) return NV_ERR_ARG;
     NV_HIP(c, hipSetDevice(c->device));
     const nv::JitKernels *k = ensure_jit(c, prec);
-    std::string t = k ? "operator-specialised (hiprtc)" : ("generic" + (c->jit_log.empty() ? std::string() : ": " + c->jit_log));
+    std::string t = k ? "operator-specialised (hiprtc)"
+                      : (std::string(nv::hv_shaped(c->dev, prec) ? "generic shaped" : "generic step-table") +
+                         (c->jit_log.empty() ? std::string() : ": " + c->jit_log));
     std::snprintf(buf, (size_t)len, "%s", t.c_str());
     return NV_OK;
 }
@@ -1647,6 +1660,11 @@ int32_t nv_batch_create(const nv_problem *probs, int64_t count, int32_t device, 
     b->d.iter = b->iter_d;
     b->d.P = (int32_t)b->ctx.size();
     b->d.Ntot = b->Ntot;
+    b->d.shaped[0] = b->d.shaped[1] = 1;
+    for (nv_ctx *c : b->ctx) {
+        if (!nv::hv_shaped(c->dev, NV_PREC_F64)) b->d.shaped[0] = 0;
+        if (!nv::hv_shaped(c->dev, NV_PREC_F32)) b->d.shaped[1] = 0;
+    }
     *out = b;
     return NV_OK;
 }

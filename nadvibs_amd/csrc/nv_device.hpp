@@ -43,6 +43,21 @@ struct NvDiagTerm {
     int32_t code[NV_MAXF];
 };
 
+// A term's steps are the product of Delta_j in {-s_j, -s_j+2, .., s_j} over its
+// modes (slot 0 slowest), less Delta = 0.  Terms of at most three modes fall
+// into NV_NSHAPES shapes by (s_0+1, s_1+1, s_2+1); the shaped H*v kernel walks
+// each shape's product as unrolled code with the per-mode factors loaded once
+// per term.  Shapes, in group order: (2) (3) (4) (2,2) (3,2) (2,3) (1,2)
+// (2,1) (2,2,2).
+#define NV_NSHAPES 9
+struct NvTermShape {
+    int32_t cm;            // coupling matrix (row of cmat), the term's NvTerm index
+    int32_t base;          // flat stride of the first step, -sum_j s_j shft_j
+    int32_t inc[3];        // 2 shft_j: stride between consecutive Delta_j
+    int32_t mode[3];       // 0-based modes of the slots
+    int32_t code[3][4];    // ftab rows of Delta_j = -s_j + 2i
+};
+
 // Everything the H*v kernels read, passed by value.
 struct NvOpDev {
     int32_t ns, M, D, N, nmax, ncodes, nterms, ndterms;
@@ -59,4 +74,6 @@ struct NvOpDev {
     const int32_t *nfunc;  // [M]
     const double *dg;      // [ns][D] state-diagonal part E(n) + D_ss(n), precomputed
     int32_t offstate;      // 1 if D couples different states (then computed per row)
+    const NvTermShape *tshape;      // [nterms] grouped by shape, term order within a group
+    int32_t tgrp[NV_NSHAPES + 1];   // first descriptor of each shape group
 };

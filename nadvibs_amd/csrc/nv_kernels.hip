@@ -281,6 +281,161 @@ __device__ __forceinline__ void offdiag_accumulate2(const NvOpDev &op, const TIn
     }
 }
 
+// ------------------------------------------------------------------------
+// Shaped off-diagonal H*v.  The same sum as offdiag_accumulate, with the
+// steps of a term generated from its shape (NvTermShape): the digits and the
+// per-mode factors f_j(Delta_j) are read once per term, the step loop is
+// unrolled, a step's factor is f_0 * f_1 * f_2 (the step kernel's product
+// order) and its offset a scalar.  Neighbour rows come through buffer
+// resources sized to the vector: an offset outside it (a step that leaves the
+// basis, whose factor is 0; a negative offset wraps past 2^31) loads 0, so
+// there is no branch per step.
+// ------------------------------------------------------------------------
+typedef float nv_f2 __attribute__((ext_vector_type(2)));
+typedef float nv_f3 __attribute__((ext_vector_type(3)));
+typedef float nv_f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBufFlags = 0x00020000;
+
+template <int NS, typename TIn> struct XRes;
+
+// interleaved FP32 copy, x[r * nv_lo_stride(NS) + s], NS <= 4: one load per row
+template <int NS> struct XRes<NS, float> {
+    static constexpr unsigned kEl = nv_lo_stride(NS) * 4;
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ XRes(const float *x, int D)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)((unsigned)D * kEl), kBufFlags)) {}
+    template <typename TAcc>
+    __device__ __forceinline__ void fma(unsigned o, TAcc h, TAcc (&w)[NS]) const {
+        constexpr int P = nv_lo_stride(NS);
+        if constexpr (P == 1) {
+            const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)o, 0, 0));
+            w[0] += h * (TAcc)v;
+        } else if constexpr (P == 2) {
+            const nv_f2 v = __builtin_bit_cast(nv_f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o, 0, 0));
+#pragma unroll
+            for (int s = 0; s < NS; ++s) w[s] += h * (TAcc)v[s];
+        } else if constexpr (P == 3) {
+            const nv_f3 v = __builtin_bit_cast(nv_f3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)o, 0, 0));
+#pragma unroll
+            for (int s = 0; s < NS; ++s) w[s] += h * (TAcc)v[s];
+        } else {
+            const nv_f4 v = __builtin_bit_cast(nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)o, 0, 0));
+#pragma unroll
+            for (int s = 0; s < NS; ++s) w[s] += h * (TAcc)v[s];
+        }
+    }
+};
+
+// state-major FP64 vector, x[s * D + r]: one resource per state
+template <int NS> struct XRes<NS, double> {
+    static constexpr unsigned kEl = 8;
+    __amdgpu_buffer_rsrc_t r[NS];
+    __device__ __forceinline__ XRes(const double *x, int D) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            r[s] = __builtin_amdgcn_make_buffer_rsrc((void *)(x + (size_t)s * D), (short)0, (int)((unsigned)D * 8u),
+                                                     kBufFlags);
+    }
+    template <typename TAcc>
+    __device__ __forceinline__ void fma(unsigned o, TAcc h, TAcc (&w)[NS]) const {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const double v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r[s], (int)o, 0, 0));
+            w[s] += h * (TAcc)v;
+        }
+    }
+};
+
+// The terms of shape group g; R rows per thread (digits of row q at
+// dig + q * M * kBlock), row q's byte offset in the vector vo[q].
+template <int NS, typename TIn, typename TAcc, int R, bool SKIP, int K0, int K1, int K2>
+__device__ __forceinline__ void shaped_group(const NvOpDev &op, const XRes<NS, TIn> &xs, const unsigned (&vo)[R],
+                                             const uint8_t *dig, const TAcc *ftab, int tid, int g,
+                                             TAcc (&acc)[R][NS]) {
+    constexpr int J1 = K1 ? K1 : 1, J2 = K2 ? K2 : 1;
+    // all slots odd: the middle of the product is Delta = 0 (diagonal part)
+    constexpr bool odd = (K0 & 1) && (K1 == 0 || (K1 & 1)) && (K2 == 0 || (K2 & 1));
+    const TAcc *__restrict__ cm = cmat_of<TAcc>(op);
+    const int t1 = op.tgrp[g + 1];
+    for (int t = op.tgrp[g]; t < t1; ++t) {
+        const NvTermShape d = op.tshape[t];
+        TAcc f0[R][K0], f1[R][J1], f2[R][J2];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const uint8_t *dq = dig + q * op.M * kBlock + tid;
+            const int n0 = dq[d.mode[0] * kBlock];
+#pragma unroll
+            for (int i = 0; i < K0; ++i) f0[q][i] = ftab[d.code[0][i] * op.nmax + n0];
+            if constexpr (K1 > 0) {
+                const int n1 = dq[d.mode[1] * kBlock];
+#pragma unroll
+                for (int i = 0; i < K1; ++i) f1[q][i] = ftab[d.code[1][i] * op.nmax + n1];
+            }
+            if constexpr (K2 > 0) {
+                const int n2 = dq[d.mode[2] * kBlock];
+#pragma unroll
+                for (int i = 0; i < K2; ++i) f2[q][i] = ftab[d.code[2][i] * op.nmax + n2];
+            }
+        }
+        TAcc w[R][NS];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) w[q][s] = (TAcc)0;
+#pragma unroll
+        for (int i0 = 0; i0 < K0; ++i0)
+#pragma unroll
+            for (int i1 = 0; i1 < J1; ++i1)
+#pragma unroll
+                for (int i2 = 0; i2 < J2; ++i2) {
+                    if (odd && 2 * i0 + 1 == K0 && (K1 == 0 || 2 * i1 + 1 == K1) && (K2 == 0 || 2 * i2 + 1 == K2))
+                        continue;
+                    int off = d.base + i0 * d.inc[0];
+                    if constexpr (K1 > 0) off += i1 * d.inc[1];
+                    if constexpr (K2 > 0) off += i2 * d.inc[2];
+                    const unsigned ob = (unsigned)off * XRes<NS, TIn>::kEl;
+#pragma unroll
+                    for (int q = 0; q < R; ++q) {
+                        TAcc h = f0[q][i0];
+                        if constexpr (K1 > 0) h *= f1[q][i1];
+                        if constexpr (K2 > 0) h *= f2[q][i2];
+                        unsigned o = vo[q] + ob;
+                        if constexpr (SKIP) o = h != (TAcc)0 ? o : 0x80000000u;
+                        xs.template fma<TAcc>(o, h, w[q]);
+                    }
+                }
+        const TAcc *C = cm + (size_t)d.cm * NS * NS;
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                TAcc a = acc[q][s];
+#pragma unroll
+                for (int s2 = 0; s2 < NS; ++s2) a += C[s * NS + s2] * w[q][s2];
+                acc[q][s] = a;
+            }
+    }
+}
+
+template <int NS, typename TIn, typename TAcc, int R, bool SKIP>
+__device__ __forceinline__ void offdiag_shaped(const NvOpDev &op, const XRes<NS, TIn> &xs, const unsigned (&vo)[R],
+                                               const uint8_t *dig, const TAcc *ftab, int tid, TAcc (&acc)[R][NS]) {
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[q][s] = (TAcc)0;
+    shaped_group<NS, TIn, TAcc, R, SKIP, 2, 0, 0>(op, xs, vo, dig, ftab, tid, 0, acc);
+    shaped_group<NS, TIn, TAcc, R, SKIP, 3, 0, 0>(op, xs, vo, dig, ftab, tid, 1, acc);
+    shaped_group<NS, TIn, TAcc, R, SKIP, 4, 0, 0>(op, xs, vo, dig, ftab, tid, 2, acc);
+    shaped_group<NS, TIn, TAcc, R, SKIP, 2, 2, 0>(op, xs, vo, dig, ftab, tid, 3, acc);
+    shaped_group<NS, TIn, TAcc, R, SKIP, 3, 2, 0>(op, xs, vo, dig, ftab, tid, 4, acc);
+    shaped_group<NS, TIn, TAcc, R, SKIP, 2, 3, 0>(op, xs, vo, dig, ftab, tid, 5, acc);
+    shaped_group<NS, TIn, TAcc, R, SKIP, 1, 2, 0>(op, xs, vo, dig, ftab, tid, 6, acc);
+    shaped_group<NS, TIn, TAcc, R, SKIP, 2, 1, 0>(op, xs, vo, dig, ftab, tid, 7, acc);
+    shaped_group<NS, TIn, TAcc, R, SKIP, 2, 2, 2>(op, xs, vo, dig, ftab, tid, 8, acc);
+}
+
 template <typename TAcc>
 __device__ __forceinline__ TAcc *stage_tables(const NvOpDev &op, uint8_t *&dig) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -403,6 +558,80 @@ __global__ __launch_bounds__(kBlock) void hv_iter2_kernel(NvOpDev op, LzState st
     }
 }
 
+// hv_iter_kernel over the shaped terms; R rows per thread.  Block b covers
+// the rows of the one-row kernel's blocks R*b .. R*b+R-1 and writes their
+// partials, so alpha is the same sum in the same order.  Rows past the end
+// compute on zero digits and out-of-range loads; they are not written.
+template <int NS, typename TIn, typename TAcc, int R, bool SKIP>
+__global__ __launch_bounds__(kBlock) void hv_shaped_iter_kernel(NvOpDev op, LzState st) {
+    uint8_t *dig;
+    const TAcc *ftab = stage_tables<TAcc>(op, dig);
+    const int tid = threadIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t it = *st.iter;
+    const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;
+    const TIn *x = st.vlo ? reinterpret_cast<const TIn *>(st.vlo) : reinterpret_cast<const TIn *>(q);
+    int r[R];
+    bool live[R];
+    unsigned vo[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        r[k] = (R * b + k) * kBlock + tid;
+        live[k] = r[k] < op.D;
+        vo[k] = (unsigned)r[k] * XRes<NS, TIn>::kEl;
+        uint8_t *dk = dig + k * op.M * kBlock;
+        if (live[k]) digits_energy(op, r[k], dk, tid);
+        else for (int m = 0; m < op.M; ++m) dk[m * kBlock + tid] = 0;
+    }
+    __syncthreads();
+    const XRes<NS, TIn> xs(x, op.D);
+    TAcc acc[R][NS];
+    offdiag_shaped<NS, TIn, TAcc, R, SKIP>(op, xs, vo, dig, ftab, tid, acc);
+    TAcc *y = reinterpret_cast<TAcc *>(st.y);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        double part = 0.0;
+        if (live[k]) {
+            double v[NS], dv[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r[k]];
+            diag_apply<NS>(op, r[k], dig + k * op.M * kBlock, tid, v, dv);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                y[(size_t)s * op.D + r[k]] = acc[k][s];
+                part += v[s] * (dv[s] + (double)acc[k][s]);
+            }
+        }
+        const double bs = block_sum(part);
+        if (tid == 0) st.partials[R * b + k] = bs;
+    }
+}
+
+// hv_full_kernel over the shaped terms.
+template <int NS, typename TAcc>
+__global__ __launch_bounds__(kBlock) void hv_shaped_full_kernel(NvOpDev op, const double *__restrict__ x,
+                                                                 double *__restrict__ y) {
+    uint8_t *dig;
+    const TAcc *ftab = stage_tables<TAcc>(op, dig);
+    const int tid = threadIdx.x;
+    const int r = xcd_block(blockIdx.x, gridDim.x) * kBlock + tid;
+    const bool live = r < op.D;
+    if (live) digits_energy(op, r, dig, tid);
+    else for (int m = 0; m < op.M; ++m) dig[m * kBlock + tid] = 0;
+    __syncthreads();
+    const XRes<NS, double> xs(x, op.D);
+    const unsigned vo[1] = {(unsigned)r * 8u};
+    TAcc acc[1][NS];
+    offdiag_shaped<NS, double, TAcc, 1, false>(op, xs, vo, dig, ftab, tid, acc);
+    if (!live) return;
+    double v[NS], dv[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) v[s] = x[(size_t)s * op.D + r];
+    diag_apply<NS>(op, r, dig, tid, v, dv);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) y[(size_t)s * op.D + r] = dv[s] + (double)acc[0][s];
+}
+
 // lanczos_step (Source/top.f90:820-842): r = (H q_i - alpha q_i) - beta(i-1) q_{i-1}
 // stored in slot i+1, beta(i) = |r|.
 template <int NS, typename TAcc>
@@ -513,9 +742,41 @@ static int grid_stride_blocks(int64_t n) {
     return (int)(b < 8192 ? b : 8192);
 }
 
+// Above this vector size the shaped kernels point the loads of out-of-basis
+// steps outside the buffer (no memory traffic) instead of at a wrong row.
+constexpr int64_t kSkipMinBytes = (int64_t)128 << 20;
+
+// The shaped kernels serve operators whose terms all have a shape (ordr <= 3),
+// up to four states, FP64 or FP32, with every row offset inside a 32-bit
+// buffer range; op.tshape = nullptr selects the step-table kernels.
+static bool shaped_ok(const NvOpDev &op, int64_t el) {
+    if (!op.tshape || op.ns > 4) return false;
+    return ((int64_t)op.D + 2 * kBlock) * el < ((int64_t)1 << 31);
+}
+
+// FP64 reads the state-major q_i, one load per state and step.  Above
+// kSkipMinBytes (cfg3: 708 MB) the step-table kernel, which skips the loads of
+// out-of-basis steps by a branch, is faster there (55.9 vs 65.7 ms per H*v).
+bool hv_shaped(const NvOpDev &op, int prec) {
+    if (prec == 0) return shaped_ok(op, 8) && (int64_t)op.D * 8 * op.ns <= kSkipMinBytes;
+    if (prec == 1) return shaped_ok(op, nv_lo_stride(op.ns) * 4);
+    return false;
+}
+
 template <int NS>
 static hipError_t hv_full_ns(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s) {
     const size_t lds = hv_lds_bytes(op, prec);
+    if constexpr (NS <= 4) {
+        if (hv_shaped(op, 0)) {
+            if (prec == 0)
+                hipLaunchKernelGGL((hv_shaped_full_kernel<NS, double>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op,
+                                   x, y);
+            else
+                hipLaunchKernelGGL((hv_shaped_full_kernel<NS, float>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op,
+                                   x, y);
+            return hipGetLastError();
+        }
+    }
     if (prec == 0)
         hipLaunchKernelGGL((hv_full_kernel<NS, double>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, x, y);
     else
@@ -531,6 +792,35 @@ constexpr int kTwoRowMinRows = 1 << 22;
 template <int NS>
 static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     const size_t lds = hv_lds_bytes(op, prec);
+    if constexpr (NS <= 4) {
+        if (hv_shaped(op, prec)) {
+            const bool two = prec == 1 && op.D >= kTwoRowMinRows;
+            const unsigned g = (unsigned)(two ? (hv_blocks(op) + 1) / 2 : hv_blocks(op));
+            const int64_t vbytes = (int64_t)op.D * (prec == 0 ? 8 * NS : nv_lo_stride(NS) * 4);
+            if (vbytes > kSkipMinBytes) {
+                if (prec == 0)
+                    hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, double, double, 1, true>), dim3(g), dim3(kBlock), lds,
+                                       s, op, st);
+                else if (two)
+                    hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, float, float, 2, true>), dim3(g), dim3(kBlock), lds,
+                                       s, op, st);
+                else
+                    hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, float, float, 1, true>), dim3(g), dim3(kBlock), lds,
+                                       s, op, st);
+                return hipGetLastError();
+            }
+            if (prec == 0)
+                hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, double, double, 1, false>), dim3(g), dim3(kBlock), lds,
+                                   s, op, st);
+            else if (two)
+                hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, float, float, 2, false>), dim3(g), dim3(kBlock), lds,
+                                   s, op, st);
+            else
+                hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, float, float, 1, false>), dim3(g), dim3(kBlock), lds,
+                                   s, op, st);
+            return hipGetLastError();
+        }
+    }
     if (prec != 0 && op.D >= kTwoRowMinRows) {
         const unsigned g = (unsigned)((hv_blocks(op) + 1) / 2);
         if (prec == 1)
@@ -1092,6 +1382,55 @@ __global__ __launch_bounds__(kBlock) void batch_hv2_kernel(BatchDev b, int t0) {
     if (tid == 0) b.partials[t] = bs;
 }
 
+// batch_hv2_kernel / batch_hv_kernel over the shaped terms: R rows per
+// thread (row k of a thread at row0 + k * blockDim.x + tid), for batches
+// whose every operator has shaped terms.
+template <int NS, typename TIn, typename TAcc, int R>
+__global__ __launch_bounds__(kBlock) void batch_shaped_kernel(BatchDev b, int t0) {
+    const int t = t0 + blockIdx.x;
+    const BatchTile tl = b.tiles[t];
+    const NvOpDev op = b.ops[tl.prob];
+    uint8_t *dig;
+    const TAcc *ftab = stage_tables<TAcc>(op, dig);
+    const int tid = threadIdx.x;
+    const int64_t it = *b.iter;
+    const double *q = b.V + slot_of(it, 3) * b.Ntot + b.voff[tl.prob];
+    const TIn *x = b.vlo ? reinterpret_cast<const TIn *>(b.vlo) + b.loff[tl.prob] : reinterpret_cast<const TIn *>(q);
+    TAcc *y = reinterpret_cast<TAcc *>(b.y) + b.voff[tl.prob];
+    int r[R];
+    bool live[R];
+    unsigned vo[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        r[k] = tl.row0 + k * (int)blockDim.x + tid;
+        live[k] = r[k] < op.D;
+        vo[k] = (unsigned)r[k] * XRes<NS, TIn>::kEl;
+        uint8_t *dk = dig + k * op.M * kBlock;
+        if (live[k]) digits_energy(op, r[k], dk, tid);
+        else for (int m = 0; m < op.M; ++m) dk[m * kBlock + tid] = 0;
+    }
+    __syncthreads();
+    const XRes<NS, TIn> xs(x, op.D);
+    TAcc acc[R][NS];
+    offdiag_shaped<NS, TIn, TAcc, R, false>(op, xs, vo, dig, ftab, tid, acc);
+    double part = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (!live[k]) continue;
+        double v[NS], dv[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r[k]];
+        diag_apply<NS>(op, r[k], dig + k * op.M * kBlock, tid, v, dv);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            y[(size_t)s * op.D + r[k]] = acc[k][s];
+            part += v[s] * (dv[s] + (double)acc[k][s]);
+        }
+    }
+    const double bs = block_sum(part);
+    if (tid == 0) b.partials[t] = bs;
+}
+
 template <int NS, typename TAcc>
 __device__ __forceinline__ void batch_step_row(const NvOpDev &op, const double *q, const double *qm, double *res,
                                                const TAcc *y, double a, double bb, int64_t it, int r,
@@ -1161,6 +1500,18 @@ hipError_t launch_batch_normalize(const BatchDev &b, int prec, hipStream_t s) {
 
 template <int NS>
 static void batch_hv_ns(const BatchDev &b, int prec, size_t lds, int t0, int nt, hipStream_t s) {
+    if constexpr (NS <= 4) {
+        if (b.shaped[prec == 0 ? 0 : 1]) {
+            if (prec == 0)
+                hipLaunchKernelGGL((batch_shaped_kernel<NS, double, double, 1>), dim3(nt), dim3(b.bs), lds, s, b, t0);
+            else if (b.bs >= 128)
+                hipLaunchKernelGGL((batch_shaped_kernel<NS, float, float, 2>), dim3(nt), dim3(b.bs / 2), lds, s, b,
+                                   t0);
+            else
+                hipLaunchKernelGGL((batch_shaped_kernel<NS, float, float, 1>), dim3(nt), dim3(b.bs), lds, s, b, t0);
+            return;
+        }
+    }
     // FP32: two rows per thread, as for large single operators (workgroups of at least one wave)
     if (prec == 0)
         hipLaunchKernelGGL((batch_hv_kernel<NS, double, double>), dim3(nt), dim3(b.bs), lds, s, b, t0);

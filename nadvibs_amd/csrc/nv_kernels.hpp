@@ -33,6 +33,9 @@ struct LzState {
 enum { kCntAlpha = 0, kCntBeta = 1, kCntPro = 2, kCntChk = 3, kCntMisc = 4, kNumCounters = 8 };
 
 size_t hv_lds_bytes(const NvOpDev &op, int prec);
+// Whether the generic H*v of precision prec runs the shaped kernels (terms
+// walked by step shape) rather than the step-table kernels.
+bool hv_shaped(const NvOpDev &op, int prec);
 int hv_blocks(const NvOpDev &op);
 
 // dg[s][r] = E(r) + D_ss(r) (once per operator).
@@ -120,6 +123,7 @@ struct BatchDev {
     double *partials = nullptr;       // [T]
     const int64_t *iter = nullptr;    // current iteration (device)
     int32_t P = 0, T = 0, bs = 256;   // problems, tiles, rows per tile (= workgroup size)
+    int32_t shaped[2] = {0, 0};       // every operator runs the shaped H*v (FP64, FP32)
 };
 // Tiles [t0, t0+nt) all belong to problems with ns states (one launch each).
 struct BatchGroup {

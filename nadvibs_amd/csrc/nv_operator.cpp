@@ -201,6 +201,11 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
     int64_t trm_rows = 0;
     op.term_dev.assign(op.terms.size(), -1);
     op.term_ddev.assign(op.terms.size(), -1);
+    // step shapes (s_0+1, s_1+1, s_2+1), in the group order of NvTermShape
+    static const int kShapes[NV_NSHAPES][3] = {{2, 0, 0}, {3, 0, 0}, {4, 0, 0}, {2, 2, 0}, {3, 2, 0},
+                                               {2, 3, 0}, {1, 2, 0}, {2, 1, 0}, {2, 2, 2}};
+    std::vector<std::vector<NvTermShape>> by_shape(NV_NSHAPES);
+    op.shaped = true;
     for (size_t ti = 0; ti < op.terms.size(); ++ti) {
         Term &t = op.terms[ti];
         std::vector<double> C(ns * ns, 0.0);
@@ -249,6 +254,33 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
             }
         }
         if (dt.nsteps > 0) {
+            int k[3] = {0, 0, 0};
+            for (int j = 0; j < t.cnt && j < 3; ++j) k[j] = t.smax[j] + 1;
+            int sid = -1;
+            if (t.cnt <= 3)
+                for (int q = 0; q < NV_NSHAPES; ++q)
+                    if (k[0] == kShapes[q][0] && k[1] == kShapes[q][1] && k[2] == kShapes[q][2]) sid = q;
+            if (sid < 0) {
+                op.shaped = false;
+            } else {
+                NvTermShape sh;
+                sh.cm = (int32_t)op.dev_terms.size();
+                int64_t base = 0;
+                for (int j = 0; j < 3; ++j) {
+                    sh.inc[j] = 0;
+                    sh.mode[j] = 0;
+                    for (int i = 0; i < 4; ++i) sh.code[j][i] = 0;
+                }
+                for (int j = 0; j < t.cnt; ++j) {
+                    base -= (int64_t)t.smax[j] * op.shft[t.mode[j]];
+                    sh.inc[j] = (int32_t)(2 * op.shft[t.mode[j]]);
+                    sh.mode[j] = t.mode[j];
+                    for (int i = 0; i <= t.smax[j]; ++i)
+                        sh.code[j][i] = code_of(t.mode[j], t.pow[j], -t.smax[j] + 2 * i);
+                }
+                sh.base = (int32_t)base;
+                by_shape[sid].push_back(sh);
+            }
             op.term_dev[ti] = (int32_t)op.dev_terms.size();
             op.dev_terms.push_back(dt);
             op.cmat.insert(op.cmat.end(), C.begin(), C.end());
@@ -268,6 +300,16 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
         }
     }
     (void)trm_rows;
+    op.dev_shapes.clear();
+    for (int q = 0; q < NV_NSHAPES; ++q) {
+        op.shape_grp[q] = (int32_t)op.dev_shapes.size();
+        op.dev_shapes.insert(op.dev_shapes.end(), by_shape[q].begin(), by_shape[q].end());
+    }
+    op.shape_grp[NV_NSHAPES] = (int32_t)op.dev_shapes.size();
+    if (!op.shaped) {
+        op.dev_shapes.clear();
+        for (int q = 0; q <= NV_NSHAPES; ++q) op.shape_grp[q] = 0;
+    }
     // constant block matrix, gated by the diagonal-block list (Source/top.f90:528-553)
     op.dconst.assign(ns * ns, 0.0);
     int napplied = 0;

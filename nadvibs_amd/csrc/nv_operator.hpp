@@ -48,6 +48,10 @@ struct Operator {
     int64_t nnz_fma = 0;                 // FMAs of one H*v (nonzero structure)
     std::vector<int32_t> term_dev;       // [terms] index into dev_terms, -1 if the term has no Delta != 0 step
     std::vector<int32_t> term_ddev;      // [terms] index into dev_dterms, -1 if it has no Delta = 0 part
+    // the off-diagonal terms by step shape (valid when every term has one)
+    bool shaped = false;
+    std::vector<NvTermShape> dev_shapes;  // [dev_terms] grouped by shape
+    int32_t shape_grp[NV_NSHAPES + 1] = {0};
 
     // start vector data
     int32_t neworigin = 0;

@@ -201,8 +201,10 @@ class Engine:
 
     # -- derivatives -----------------------------------------------------------
     def select_hv_kernel(self, generic=True):
-        """generic=True skips the run-time compile of the specialised H*v."""
-        self._check(self.lib.nv_select_hv_kernel(self.h, 1 if generic else 0))
+        """generic=True skips the run-time compile of the specialised H*v;
+        generic="steps" also keeps the generic kernel in its step-table form."""
+        code = 2 if generic == "steps" else (1 if generic else 0)
+        self._check(self.lib.nv_select_hv_kernel(self.h, code))
 
     def lanczos_adjoint(self, abar, bbar, prec="f64"):
         """dL/d(aomega, concoef, potterms) of the last run, given

@@ -1,7 +1,7 @@
 """GPU: batched runs (nv_batch_*) against single-problem runs and the oracle.
 
 Every problem of a batch must reproduce its own single-problem run: the same
-per-row H*v arithmetic (table-driven kernel) and the same recurrence, with the
+per-row H*v arithmetic (generic kernel) and the same recurrence, with the
 dots summed over different partitions, so alpha/beta agree to 1e-11
 relative over 40 iterations in FP64 (1e-8 in FP32); against the oracle the
 first 20 coefficients agree to 1e-10.
@@ -31,8 +31,15 @@ def _single(p, n, prec):
         return eng.coefficients()[:2]
 
 
+@pytest.mark.parametrize("form", ["shaped", "steps"])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-11), ("f32", 1e-8)])
-def test_batch_matches_single_runs(prec, tol):
+def test_batch_matches_single_runs(prec, tol, form, monkeypatch):
+    """Both forms of the generic H*v: terms walked by step shape, and the
+    step table (NV_HV_KERNEL=steps, for batch and single runs alike)."""
+    if form == "steps":
+        monkeypatch.setenv("NV_HV_KERNEL", "steps")
+    else:
+        monkeypatch.delenv("NV_HV_KERNEL", raising=False)
     probs = _probs()
     n = 40
     with Batch(probs) as bt:

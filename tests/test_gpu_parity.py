@@ -36,21 +36,28 @@ def _prob(case, seed=11):
                         quirks=quirks, offdiag_even=offe)
 
 
-@pytest.fixture(params=["specialised", "generic"])
+@pytest.fixture(params=["specialised", "generic", "steps"])
 def hv_kernel(request, monkeypatch):
-    """Both H*v kernels: the operator-specialised one (compiled at run time)
-    and the generic one (NV_HV_KERNEL=generic)."""
-    if request.param == "generic":
-        monkeypatch.setenv("NV_HV_KERNEL", "generic")
-    else:
+    """Every H*v kernel: the operator-specialised one (compiled at run time),
+    the generic one (NV_HV_KERNEL=generic; terms walked by step shape where
+    the operator allows) and the generic step-table one (NV_HV_KERNEL=steps)."""
+    if request.param == "specialised":
         monkeypatch.delenv("NV_HV_KERNEL", raising=False)
+    else:
+        monkeypatch.setenv("NV_HV_KERNEL", request.param)
     return request.param
 
 
 def _check_kernel(eng, kind, prec):
     info = eng.hv_kernel(prec)
     print(f"H*v kernel ({prec}): {info}", flush=True)
-    assert info.startswith("operator-specialised" if kind == "specialised" else "generic"), info
+    if kind == "specialised":
+        assert info.startswith("operator-specialised"), info
+    elif kind == "steps":
+        assert info.startswith("generic step-table"), info
+    else:
+        shaped = eng.prob.ordr <= 3 and eng.prob.nstates <= 4
+        assert info.startswith("generic shaped" if shaped else "generic step-table"), info
 
 
 @pytest.mark.parametrize("case", CASES, ids=[f"c{i}" for i in range(len(CASES))])
@@ -130,15 +137,18 @@ def test_lanczos_bf16_input_tracks_fp64():
     assert rel.max() > 1e-7  # it is not running the FP32 path
 
 
-def test_generic_two_row_kernel_on_a_large_operator():
-    """4,194,304 rows in FP32: the table-driven H*v runs two rows per thread
-    (large operators); its first coefficients agree with the FP64 oracle to
-    the FP32 level, as the one-row kernel does on cfg1."""
+@pytest.mark.parametrize("form", ["shaped", "steps"])
+def test_generic_two_row_kernel_on_a_large_operator(form):
+    """4,194,304 rows in FP32: the generic H*v runs two rows per thread
+    (large operators), in both of its forms; its first coefficients agree
+    with the FP64 oracle to the FP32 level, as the one-row kernel does on
+    cfg1."""
     p = make_problem(2, [16, 16, 16, 16, 16, 4], 2, (400, 3000), 250.0, 60.0, 40.0, seed=13)
     assert p.n // p.nstates == 1 << 22
     ref = Oracle(p).lanczos(3)
     with Engine(p) as eng:
-        eng.select_hv_kernel(True)
+        eng.select_hv_kernel(True if form == "shaped" else "steps")
+        assert eng.hv_kernel("f32").startswith("generic " + ("shaped" if form == "shaped" else "step-table"))
         eng.lanczos_init(niter=3, prec="f32")
         eng.run(1, 3)
         a, b, _ = eng.coefficients()
