@@ -94,7 +94,7 @@ def cpu_baseline(prob, seconds, seed=12345):
 def batched_cfg5(sync, cpu_seconds):
     """The uneven 64-problem batch (400 iterations each, FP32 H*v, FP64
     recurrence): one batched run against the same problems run one after the
-    other through single contexts with the same table-driven kernels, and the
+    other through single contexts with the same generic kernels, and the
     FP64 oracle on the host over a bounded sample."""
     from nadvibs_amd.batch import Batch
     from nadvibs_amd.engine import Engine
@@ -188,7 +188,7 @@ def adjoint_cfg4(sync):
     pt0 = torch.tensor(p.potterms, dtype=torch.float64)
     scale = 1 + 0.05 * torch.from_numpy(rng.standard_normal(pt0.shape))
     res = {"workload": "cfg4: cfg2 operator, 500 iterations, sigma 15 cm^-1, 4096 points",
-           "gradient": "Krylov-projected (nv_spectrum_adjoint + nv_krylov_gradient), table-driven H*v"}
+           "gradient": "Krylov-projected (nv_spectrum_adjoint + nv_krylov_gradient), generic H*v"}
     runs = (("checkpoints_50", dict(reorthog=0, checkpoint=50)), ("pro_kept_basis", dict(reorthog=1)))
     for label, hi in (("window_0_6000", 6000.0), ("window_0_1500", 1500.0)):
         grid = torch.linspace(0.0, hi, 4096, dtype=torch.float64)
@@ -223,8 +223,9 @@ def cfg3_probe(sync, iters=8):
     """cfg3 (nstates 4, nmodes 10, basis [8,8,6,6,6,5,5,4,4,4], N = 88,473,600,
     ordr 3, reorthog 0): the first iterations in FP64 and with FP32 H*v, and
     the H*v kernel alone (back-to-back launches).  The 1760-step operator runs
-    through the table-driven kernel.  The full 3000-iteration job is not run
-    here; its time is the per-iteration figure times 3000."""
+    through the generic kernel (shaped in FP32, step table in FP64).  The full
+    3000-iteration job is not run here; its time is the per-iteration figure
+    times 3000."""
     from nadvibs_amd.engine import Engine
     from nadvibs_amd.synth import workload
     p = workload("cfg3")
@@ -320,6 +321,7 @@ def pmc_probe(args):
         dst.copy_(src)
     torch.cuda.synchronize()
     with Engine(prob, device=0) as eng:
+        kernel = eng.hv_kernel("f32")
         eng.lanczos_init(niter=4, reorthog=0, prec="f32")
         eng.update_lanczos(1)
         for _ in range(args.steps):
@@ -327,7 +329,7 @@ def pmc_probe(args):
         eng.lanczos_step(1)
         eng.coefficients(n=1)
     print(json.dumps({"copy_bytes": 64 * 1024 * 1024, "hv_launches": args.steps, "n": prob.n,
-                      "hv_kernel": "nv_hv_iter_f32"}), flush=True)
+                      "hv_kernel": kernel}), flush=True)
 
 
 def main():

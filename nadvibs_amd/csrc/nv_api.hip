@@ -179,6 +179,10 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
     {
         const char *k = std::getenv("NV_HV_KERNEL");
         if (k && std::strcmp(k, "steps") == 0) d.tshape = nullptr;
+        // NV_HV_SKIP_MIN_BYTES: vector size from which the shaped H*v skips
+        // the loads of out-of-basis steps (0: always)
+        const char *m = std::getenv("NV_HV_SKIP_MIN_BYTES");
+        d.skip_min_bytes = m ? std::strtoll(m, nullptr, 10) : nv::hv_skip_min_bytes();
     }
     if (e != hipSuccess) {
         g_create_error = std::string("device setup failed: ") + hipGetErrorString(e);

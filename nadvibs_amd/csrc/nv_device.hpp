@@ -76,4 +76,5 @@ struct NvOpDev {
     int32_t offstate;      // 1 if D couples different states (then computed per row)
     const NvTermShape *tshape;      // [nterms] grouped by shape, term order within a group
     int32_t tgrp[NV_NSHAPES + 1];   // first descriptor of each shape group
+    int64_t skip_min_bytes;         // shaped H*v: vector size above which out-of-basis loads are skipped
 };

@@ -742,8 +742,9 @@ static int grid_stride_blocks(int64_t n) {
     return (int)(b < 8192 ? b : 8192);
 }
 
-// Above this vector size the shaped kernels point the loads of out-of-basis
-// steps outside the buffer (no memory traffic) instead of at a wrong row.
+// Above op.skip_min_bytes of vector (default kSkipMinBytes) the shaped
+// kernels point the loads of out-of-basis steps outside the buffer (no memory
+// traffic) instead of at a wrong row.  FP64 runs shaped up to kSkipMinBytes.
 constexpr int64_t kSkipMinBytes = (int64_t)128 << 20;
 
 // The shaped kernels serve operators whose terms all have a shape (ordr <= 3),
@@ -757,6 +758,8 @@ static bool shaped_ok(const NvOpDev &op, int64_t el) {
 // FP64 reads the state-major q_i, one load per state and step.  Above
 // kSkipMinBytes (cfg3: 708 MB) the step-table kernel, which skips the loads of
 // out-of-basis steps by a branch, is faster there (55.9 vs 65.7 ms per H*v).
+int64_t hv_skip_min_bytes() { return kSkipMinBytes; }
+
 bool hv_shaped(const NvOpDev &op, int prec) {
     if (prec == 0) return shaped_ok(op, 8) && (int64_t)op.D * 8 * op.ns <= kSkipMinBytes;
     if (prec == 1) return shaped_ok(op, nv_lo_stride(op.ns) * 4);
@@ -797,7 +800,7 @@ static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hip
             const bool two = prec == 1 && op.D >= kTwoRowMinRows;
             const unsigned g = (unsigned)(two ? (hv_blocks(op) + 1) / 2 : hv_blocks(op));
             const int64_t vbytes = (int64_t)op.D * (prec == 0 ? 8 * NS : nv_lo_stride(NS) * 4);
-            if (vbytes > kSkipMinBytes) {
+            if (vbytes > op.skip_min_bytes) {
                 if (prec == 0)
                     hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, double, double, 1, true>), dim3(g), dim3(kBlock), lds,
                                        s, op, st);

@@ -36,6 +36,8 @@ size_t hv_lds_bytes(const NvOpDev &op, int prec);
 // Whether the generic H*v of precision prec runs the shaped kernels (terms
 // walked by step shape) rather than the step-table kernels.
 bool hv_shaped(const NvOpDev &op, int prec);
+// Default of NvOpDev::skip_min_bytes (128 MB).
+int64_t hv_skip_min_bytes();
 int hv_blocks(const NvOpDev &op);
 
 // dg[s][r] = E(r) + D_ss(r) (once per operator).

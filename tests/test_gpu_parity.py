@@ -36,13 +36,20 @@ def _prob(case, seed=11):
                         quirks=quirks, offdiag_even=offe)
 
 
-@pytest.fixture(params=["specialised", "generic", "steps"])
+@pytest.fixture(params=["specialised", "generic", "skip", "steps"])
 def hv_kernel(request, monkeypatch):
     """Every H*v kernel: the operator-specialised one (compiled at run time),
     the generic one (NV_HV_KERNEL=generic; terms walked by step shape where
-    the operator allows) and the generic step-table one (NV_HV_KERNEL=steps)."""
+    the operator allows), the same with the loads of out-of-basis steps
+    pointed outside the vector (the large-vector form, here forced by
+    NV_HV_SKIP_MIN_BYTES=0) and the generic step-table one
+    (NV_HV_KERNEL=steps)."""
+    monkeypatch.delenv("NV_HV_SKIP_MIN_BYTES", raising=False)
     if request.param == "specialised":
         monkeypatch.delenv("NV_HV_KERNEL", raising=False)
+    elif request.param == "skip":
+        monkeypatch.setenv("NV_HV_KERNEL", "generic")
+        monkeypatch.setenv("NV_HV_SKIP_MIN_BYTES", "0")
     else:
         monkeypatch.setenv("NV_HV_KERNEL", request.param)
     return request.param
