@@ -404,6 +404,24 @@ def main():
             sync()
             plain[p] = args.niter / (time.perf_counter() - t0)
         print(f"# reorthog 0: {plain}", file=sys.stderr, flush=True)
+        # the same job with the H*v input rounded to BF16 (8-bit mantissa, FP32 arithmetic):
+        # its eigenvalue error against FP64, beside the FP32 input's
+        try:
+            run_job(eng, args.niter, "bf16", args.reorthog, args.chkorthog, args.seed)
+            sync()
+            t0 = time.perf_counter()
+            outbf = run_job(eng, args.niter, "bf16", args.reorthog, args.chkorthog, args.seed)
+            sync()
+            bf16 = (args.niter / (time.perf_counter() - t0), outbf)
+        except RuntimeError as e:
+            bf16 = str(e)
+        print(f"# bf16 input: {bf16 if isinstance(bf16, str) else round(bf16[0], 1)}", file=sys.stderr, flush=True)
+        # the first coefficients of BF16 and FP32 input against FP64 (no re-orthogonalisation)
+        first = {}
+        for p in ("f64", "f32", "bf16"):
+            eng.lanczos_init(niter=50, reorthog=0, prec=p, seed=args.seed)
+            eng.run(1, 50)
+            first[p] = eng.coefficients()[0]
 
     iters = args.steps * args.niter
     value = iters / dt32
@@ -443,6 +461,19 @@ def main():
         rec["max_abs_dI"] = float(f"{dI:.3e}")
         rec["n_converged_fp64"] = nconv
         rec["error_target_met"] = bool(dE <= 1e-3)
+    if res64 is not None:
+        a_rel = {p: float(f"{np.max(np.abs(first[p] - first['f64']) / np.abs(first['f64'])):.3e}")
+                 for p in ("f32", "bf16")}
+        if isinstance(bf16, str):
+            rec["bf16_input"] = {"failed": bf16, "alpha_max_rel_diff_first_50": a_rel,
+                                 "note": "H*v input rounded to BF16, FP32 arithmetic, FP64 diagonal and recurrence"}
+        else:
+            out64 = res64[3]
+            dEb, dIb, _ = compare(out64[2], out64[3], out64[4], bf16[1][2], bf16[1][3], bf16[1][4])
+            rec["bf16_input"] = {"iter_per_s": round(bf16[0], 1), "max_abs_dE_cm1": float(f"{dEb:.3e}"),
+                                 "max_abs_dI": float(f"{dIb:.3e}"), "error_target_met": bool(dEb <= 1e-3),
+                                 "alpha_max_rel_diff_first_50": a_rel,
+                                 "note": "H*v input rounded to BF16, FP32 arithmetic, FP64 diagonal and recurrence"}
     rec["reorthog0"] = {"f32_iter_per_s": round(plain["f32"], 1), "f64_iter_per_s": round(plain["f64"], 1),
                         "speedup_f32_vs_f64": round(plain["f32"] / plain["f64"], 3),
                         "note": "same job without partial re-orthogonalisation (ghost copies are not compared)"}

@@ -279,7 +279,7 @@ int32_t nv_krylov_gradient(nv_ctx *ctx, int64_t n, const double *tbar,
 /* generic != 0: use the generic H*v kernel instead of compiling an
  * operator-specialised one (saves the run-time compile for operators used
  * once, e.g. inside a fit).  The generic kernel walks the terms by step shape
- * where it can (ordr <= 3, nstates <= 4); generic == 2 keeps the step-table
+ * where it can (ordr <= 3); generic == 2 keeps the step-table
  * form for every operator.  Call before nv_lanczos_init. */
 int32_t nv_select_hv_kernel(nv_ctx *ctx, int32_t generic);
 

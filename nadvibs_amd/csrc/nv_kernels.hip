@@ -299,7 +299,8 @@ constexpr int kBufFlags = 0x00020000;
 
 template <int NS, typename TIn> struct XRes;
 
-// interleaved FP32 copy, x[r * nv_lo_stride(NS) + s], NS <= 4: one load per row
+// interleaved FP32 copy, x[r * nv_lo_stride(NS) + s]: one load per row up to
+// four states, 16-byte loads of the states present beyond
 template <int NS> struct XRes<NS, float> {
     static constexpr unsigned kEl = nv_lo_stride(NS) * 4;
     __amdgpu_buffer_rsrc_t r;
@@ -319,10 +320,18 @@ template <int NS> struct XRes<NS, float> {
             const nv_f3 v = __builtin_bit_cast(nv_f3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)o, 0, 0));
 #pragma unroll
             for (int s = 0; s < NS; ++s) w[s] += h * (TAcc)v[s];
-        } else {
+        } else if constexpr (P == 4) {
             const nv_f4 v = __builtin_bit_cast(nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)o, 0, 0));
 #pragma unroll
             for (int s = 0; s < NS; ++s) w[s] += h * (TAcc)v[s];
+        } else {
+#pragma unroll
+            for (int part = 0; part * 4 < NS; ++part) {
+                const nv_f4 v = __builtin_bit_cast(
+                    nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(o + 16u * part), 0, 0));
+#pragma unroll
+                for (int s = part * 4; s < NS && s < part * 4 + 4; ++s) w[s] += h * (TAcc)v[s - part * 4];
+            }
         }
     }
 };
@@ -746,12 +755,13 @@ static int grid_stride_blocks(int64_t n) {
 // kernels point the loads of out-of-basis steps outside the buffer (no memory
 // traffic) instead of at a wrong row.  FP64 runs shaped up to kSkipMinBytes.
 constexpr int64_t kSkipMinBytes = (int64_t)128 << 20;
+constexpr int kShapedMaxStates = NV_MAXS;
 
 // The shaped kernels serve operators whose terms all have a shape (ordr <= 3),
-// up to four states, FP64 or FP32, with every row offset inside a 32-bit
+// any number of states (up to NV_MAXS), FP64 or FP32, with every row offset inside a 32-bit
 // buffer range; op.tshape = nullptr selects the step-table kernels.
 static bool shaped_ok(const NvOpDev &op, int64_t el) {
-    if (!op.tshape || op.ns > 4) return false;
+    if (!op.tshape || op.ns > kShapedMaxStates) return false;
     return ((int64_t)op.D + 2 * kBlock) * el < ((int64_t)1 << 31);
 }
 
@@ -769,7 +779,7 @@ bool hv_shaped(const NvOpDev &op, int prec) {
 template <int NS>
 static hipError_t hv_full_ns(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s) {
     const size_t lds = hv_lds_bytes(op, prec);
-    if constexpr (NS <= 4) {
+    if constexpr (NS <= kShapedMaxStates) {
         if (hv_shaped(op, 0)) {
             if (prec == 0)
                 hipLaunchKernelGGL((hv_shaped_full_kernel<NS, double>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op,
@@ -795,7 +805,7 @@ constexpr int kTwoRowMinRows = 1 << 22;
 template <int NS>
 static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     const size_t lds = hv_lds_bytes(op, prec);
-    if constexpr (NS <= 4) {
+    if constexpr (NS <= kShapedMaxStates) {
         if (hv_shaped(op, prec)) {
             const bool two = prec == 1 && op.D >= kTwoRowMinRows;
             const unsigned g = (unsigned)(two ? (hv_blocks(op) + 1) / 2 : hv_blocks(op));
@@ -1503,7 +1513,7 @@ hipError_t launch_batch_normalize(const BatchDev &b, int prec, hipStream_t s) {
 
 template <int NS>
 static void batch_hv_ns(const BatchDev &b, int prec, size_t lds, int t0, int nt, hipStream_t s) {
-    if constexpr (NS <= 4) {
+    if constexpr (NS <= kShapedMaxStates) {
         if (b.shaped[prec == 0 ? 0 : 1]) {
             if (prec == 0)
                 hipLaunchKernelGGL((batch_shaped_kernel<NS, double, double, 1>), dim3(nt), dim3(b.bs), lds, s, b, t0);

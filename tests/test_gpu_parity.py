@@ -27,6 +27,8 @@ CASES = [
     (2, [5, 1, 4], 3, 20.0, 0.0, 20.0, 3),
     (4, [3, 3, 3, 2], 2, 0.0, 0.0, 10.0, 3),
     (2, [2, 2, 2, 2, 2, 2, 2], 3, 5.0, 0.0, 0.0, 3),
+    (6, [3, 3, 2], 2, 0.0, 0.0, 10.0, 3),
+    (9, [2, 3, 2], 3, 5.0, 0.0, 0.0, 3),
 ]
 
 
@@ -63,7 +65,7 @@ def _check_kernel(eng, kind, prec):
     elif kind == "steps":
         assert info.startswith("generic step-table"), info
     else:
-        shaped = eng.prob.ordr <= 3 and eng.prob.nstates <= 4
+        shaped = eng.prob.ordr <= 3
         assert info.startswith("generic shaped" if shaped else "generic step-table"), info
 
 
