@@ -416,11 +416,12 @@ def main():
         except RuntimeError as e:
             bf16 = str(e)
         print(f"# bf16 input: {bf16 if isinstance(bf16, str) else round(bf16[0], 1)}", file=sys.stderr, flush=True)
-        # the first coefficients of BF16 and FP32 input against FP64 (no re-orthogonalisation)
+        # the first coefficients of BF16 and FP32 input against FP64 (no re-orthogonalisation;
+        # past a few tens of iterations rounding differences of any kind grow)
         first = {}
         for p in ("f64", "f32", "bf16"):
-            eng.lanczos_init(niter=50, reorthog=0, prec=p, seed=args.seed)
-            eng.run(1, 50)
+            eng.lanczos_init(niter=10, reorthog=0, prec=p, seed=args.seed)
+            eng.run(1, 10)
             first[p] = eng.coefficients()[0]
 
     iters = args.steps * args.niter
@@ -465,14 +466,14 @@ def main():
         a_rel = {p: float(f"{np.max(np.abs(first[p] - first['f64']) / np.abs(first['f64'])):.3e}")
                  for p in ("f32", "bf16")}
         if isinstance(bf16, str):
-            rec["bf16_input"] = {"failed": bf16, "alpha_max_rel_diff_first_50": a_rel,
+            rec["bf16_input"] = {"failed": bf16, "alpha_max_rel_diff_first_10": a_rel,
                                  "note": "H*v input rounded to BF16, FP32 arithmetic, FP64 diagonal and recurrence"}
         else:
             out64 = res64[3]
             dEb, dIb, _ = compare(out64[2], out64[3], out64[4], bf16[1][2], bf16[1][3], bf16[1][4])
             rec["bf16_input"] = {"iter_per_s": round(bf16[0], 1), "max_abs_dE_cm1": float(f"{dEb:.3e}"),
                                  "max_abs_dI": float(f"{dIb:.3e}"), "error_target_met": bool(dEb <= 1e-3),
-                                 "alpha_max_rel_diff_first_50": a_rel,
+                                 "alpha_max_rel_diff_first_10": a_rel,
                                  "note": "H*v input rounded to BF16, FP32 arithmetic, FP64 diagonal and recurrence"}
     rec["reorthog0"] = {"f32_iter_per_s": round(plain["f32"], 1), "f64_iter_per_s": round(plain["f64"], 1),
                         "speedup_f32_vs_f64": round(plain["f32"] / plain["f64"], 3),

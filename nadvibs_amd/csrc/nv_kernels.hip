@@ -356,6 +356,32 @@ template <int NS> struct XRes<NS, double> {
     }
 };
 
+// interleaved FP64 copy, x[r * nv_lo_stride(NS) + s]: 16-byte loads of two states
+struct F64I {};
+template <int NS> struct XRes<NS, F64I> {
+    static constexpr unsigned kEl = nv_lo_stride(NS) * 8;
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ XRes(const double *x, int D)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)((unsigned)D * kEl), kBufFlags)) {}
+    template <typename TAcc>
+    __device__ __forceinline__ void fma(unsigned o, TAcc h, TAcc (&w)[NS]) const {
+        typedef double nv_d2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int part = 0; 2 * part < NS; ++part) {
+            if (2 * part + 1 < NS) {
+                const nv_d2 v = __builtin_bit_cast(
+                    nv_d2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(o + 16u * part), 0, 0));
+                w[2 * part] += h * (TAcc)v[0];
+                w[2 * part + 1] += h * (TAcc)v[1];
+            } else {
+                const double v = __builtin_bit_cast(
+                    double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(o + 16u * part), 0, 0));
+                w[2 * part] += h * (TAcc)v;
+            }
+        }
+    }
+};
+
 // The terms of shape group g; R rows per thread (digits of row q at
 // dig + q * M * kBlock), row q's byte offset in the vector vo[q].
 template <int NS, typename TIn, typename TAcc, int R, bool SKIP, int K0, int K1, int K2>
@@ -579,7 +605,10 @@ __global__ __launch_bounds__(kBlock) void hv_shaped_iter_kernel(NvOpDev op, LzSt
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int64_t it = *st.iter;
     const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;
-    const TIn *x = st.vlo ? reinterpret_cast<const TIn *>(st.vlo) : reinterpret_cast<const TIn *>(q);
+    using TX = typename std::conditional<std::is_same<TIn, F64I>::value, double, TIn>::type;
+    const TX *x = std::is_same<TIn, F64I>::value ? reinterpret_cast<const TX *>(st.v64i)
+                  : st.vlo                       ? reinterpret_cast<const TX *>(st.vlo)
+                                                 : reinterpret_cast<const TX *>(q);
     int r[R];
     bool live[R];
     unsigned vo[R];
@@ -687,9 +716,10 @@ __global__ __launch_bounds__(kBlock) void normalize_kernel(int64_t N, int64_t D,
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock) {
         const double v = q[i] * sc;
         q[i] = v;
-        if (lo) {
+        if (lo || st.v64i) {
             const int64_t s = i / D;
-            lo[(i - s * D) * nsp + s] = from_d<TIn>(v);
+            if (lo) lo[(i - s * D) * nsp + s] = from_d<TIn>(v);
+            if (st.v64i) st.v64i[(i - s * D) * nsp + s] = v;
         }
         if (st.Vf) st.Vf[slot_of(it, st.nslots) * st.ldf + i] = (float)v;
     }
@@ -771,16 +801,23 @@ static bool shaped_ok(const NvOpDev &op, int64_t el) {
 int64_t hv_skip_min_bytes() { return kSkipMinBytes; }
 
 bool hv_shaped(const NvOpDev &op, int prec) {
-    if (prec == 0) return shaped_ok(op, 8) && (int64_t)op.D * 8 * op.ns <= kSkipMinBytes;
+    if (prec == 0) return shaped_ok(op, nv_lo_stride(op.ns) * 8);
     if (prec == 1) return shaped_ok(op, nv_lo_stride(op.ns) * 4);
     return false;
 }
+
+// Standalone and batched H*v read the state-major FP64 vector itself.
+static bool shaped_state_major(const NvOpDev &op) {
+    return shaped_ok(op, 8) && (int64_t)op.D * 8 * op.ns <= kSkipMinBytes;
+}
+
+bool hv_shaped_batch(const NvOpDev &op, int prec) { return prec == 0 ? shaped_state_major(op) : hv_shaped(op, prec); }
 
 template <int NS>
 static hipError_t hv_full_ns(const NvOpDev &op, int prec, const double *x, double *y, hipStream_t s) {
     const size_t lds = hv_lds_bytes(op, prec);
     if constexpr (NS <= kShapedMaxStates) {
-        if (hv_shaped(op, 0)) {
+        if (shaped_state_major(op)) {
             if (prec == 0)
                 hipLaunchKernelGGL((hv_shaped_full_kernel<NS, double>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op,
                                    x, y);
@@ -806,13 +843,14 @@ template <int NS>
 static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     const size_t lds = hv_lds_bytes(op, prec);
     if constexpr (NS <= kShapedMaxStates) {
-        if (hv_shaped(op, prec)) {
+        // FP64 runs shaped when the caller keeps the interleaved copy
+        if (hv_shaped(op, prec) && (prec == 1 || st.v64i)) {
             const bool two = prec == 1 && op.D >= kTwoRowMinRows;
             const unsigned g = (unsigned)(two ? (hv_blocks(op) + 1) / 2 : hv_blocks(op));
-            const int64_t vbytes = (int64_t)op.D * (prec == 0 ? 8 * NS : nv_lo_stride(NS) * 4);
+            const int64_t vbytes = (int64_t)op.D * nv_lo_stride(NS) * (prec == 0 ? 8 : 4);
             if (vbytes > op.skip_min_bytes) {
                 if (prec == 0)
-                    hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, double, double, 1, true>), dim3(g), dim3(kBlock), lds,
+                    hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, F64I, double, 1, true>), dim3(g), dim3(kBlock), lds,
                                        s, op, st);
                 else if (two)
                     hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, float, float, 2, true>), dim3(g), dim3(kBlock), lds,
@@ -823,7 +861,7 @@ static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hip
                 return hipGetLastError();
             }
             if (prec == 0)
-                hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, double, double, 1, false>), dim3(g), dim3(kBlock), lds,
+                hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, F64I, double, 1, false>), dim3(g), dim3(kBlock), lds,
                                    s, op, st);
             else if (two)
                 hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, float, float, 2, false>), dim3(g), dim3(kBlock), lds,

@@ -27,6 +27,8 @@ struct LzState {
                                  // archived q_j (slot j % nslots, state-major, slot stride ldf),
                                  // read by the overlap dots and the Gram-Schmidt updates
     int64_t ldf = 0;             // N rounded up to even (pairs of FP32 elements load as one)
+    double *v64i = nullptr;      // FP64 runs of the shaped generic H*v: q_i interleaved
+                                 // [D][nv_lo_stride(ns)], padding zero, written by normalize
 };
 
 // counters[] slots
@@ -34,8 +36,12 @@ enum { kCntAlpha = 0, kCntBeta = 1, kCntPro = 2, kCntChk = 3, kCntMisc = 4, kNum
 
 size_t hv_lds_bytes(const NvOpDev &op, int prec);
 // Whether the generic H*v of precision prec runs the shaped kernels (terms
-// walked by step shape) rather than the step-table kernels.
+// walked by step shape) rather than the step-table kernels.  In FP64 the
+// Lanczos iteration then reads the interleaved copy LzState::v64i, which the
+// caller allocates (D * nv_lo_stride(ns) doubles, zeroed).
 bool hv_shaped(const NvOpDev &op, int prec);
+// The same for the batched H*v (which reads the state-major FP64 vector).
+bool hv_shaped_batch(const NvOpDev &op, int prec);
 // Default of NvOpDev::skip_min_bytes (128 MB).
 int64_t hv_skip_min_bytes();
 int hv_blocks(const NvOpDev &op);
