@@ -295,6 +295,8 @@ typedef float nv_f2 __attribute__((ext_vector_type(2)));
 typedef float nv_f3 __attribute__((ext_vector_type(3)));
 typedef float nv_f4 __attribute__((ext_vector_type(4)));
 
+// Word 3 of the buffer resource: data format 32 (bits 15-18 = 4); the
+// run-time compiled kernels use the same value.
 constexpr int kBufFlags = 0x00020000;
 
 template <int NS, typename TIn> struct XRes;
@@ -383,7 +385,9 @@ template <int NS> struct XRes<NS, F64I> {
 };
 
 // The terms of shape group g; R rows per thread (digits of row q at
-// dig + q * M * kBlock), row q's byte offset in the vector vo[q].
+// dig + q * M * kBlock), row q's byte offset in the vector vo[q].  SKIP
+// points the load of a step whose factor is 0 outside the buffer (it then
+// moves no memory), where a plain out-of-basis offset may land on another row.
 template <int NS, typename TIn, typename TAcc, int R, bool SKIP, int K0, int K1, int K2>
 __device__ __forceinline__ void shaped_group(const NvOpDev &op, const XRes<NS, TIn> &xs, const unsigned (&vo)[R],
                                              const uint8_t *dig, const TAcc *ftab, int tid, int g,
