@@ -132,6 +132,47 @@ def test_lanczos_coefficients_cfg1(prec, tol, hv_kernel):
     np.testing.assert_allclose(b[:k], ref["beta"][:k], rtol=tol * 10)
 
 
+def _random_cases(n, seed=2024):
+    """Small operators of random shape: 1-10 states, 1-5 modes of 1-6
+    functions, ordr 1-3, either quirk set, with or without inter-state
+    even couplings."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        nf = [int(v) for v in rng.integers(1, 7, size=int(rng.integers(1, 6)))]
+        ns = int(rng.integers(1, 11))
+        if ns * int(np.prod(nf)) < 8 or ns * int(np.prod(nf)) > 3000:
+            continue
+        out.append((ns, nf, int(rng.integers(1, 4)), float(rng.choice([0.0, 10.0])), 0.0,
+                    float(rng.choice([0.0, 20.0])), int(rng.choice([0, 3]))))
+    return out
+
+
+@pytest.mark.parametrize("form", ["generic", "skip", "steps"])
+def test_random_operators_every_generic_form(form, monkeypatch):
+    """Ten random small operators through each form of the generic kernel:
+    the first Lanczos coefficients in FP64 and with FP32 H*v against the
+    oracle (the shaped forms cover every term shape, state count up to ten,
+    modes of one function and both quirk sets)."""
+    monkeypatch.setenv("NV_HV_KERNEL", "generic" if form == "skip" else form)
+    if form == "skip":
+        monkeypatch.setenv("NV_HV_SKIP_MIN_BYTES", "0")
+    else:
+        monkeypatch.delenv("NV_HV_SKIP_MIN_BYTES", raising=False)
+    for i, case in enumerate(_random_cases(10)):
+        p = _prob(case, seed=300 + i)
+        k = min(6, max(2, p.n // 4))
+        print(f"random case {i}: {case} n={p.n} k={k}", flush=True)
+        ref = Oracle(p).lanczos(k)
+        with Engine(p) as eng:
+            for prec, tol in (("f64", 1e-11), ("f32", 5e-6)):
+                eng.lanczos_init(niter=k, prec=prec)
+                eng.run(1, k)
+                a, b, _ = eng.coefficients()
+                np.testing.assert_allclose(a, ref["alpha"], rtol=tol, atol=1e-14)
+                np.testing.assert_allclose(b[:k], ref["beta"][:k], rtol=tol * 10, atol=1e-14)
+
+
 def test_lanczos_bf16_input_tracks_fp64():
     """BF16 copy of the H*v input (8-bit mantissa, FP32 arithmetic, FP64
     diagonal and recurrence): the first coefficients follow the FP64 oracle
