@@ -192,10 +192,9 @@ def test_lanczos_bf16_input_tracks_fp64():
 
 @pytest.mark.parametrize("form", ["shaped", "steps"])
 def test_generic_two_row_kernel_on_a_large_operator(form):
-    """4,194,304 rows in FP32: the generic H*v runs two rows per thread
-    (large operators), in both of its forms; its first coefficients agree
-    with the FP64 oracle to the FP32 level, as the one-row kernel does on
-    cfg1."""
+    """4,194,304 rows in FP32, the generic H*v in both of its forms (the
+    step-table form runs two rows per thread from this size on): its first
+    coefficients agree with the FP64 oracle to the FP32 level, as on cfg1."""
     p = make_problem(2, [16, 16, 16, 16, 16, 4], 2, (400, 3000), 250.0, 60.0, 40.0, seed=13)
     assert p.n // p.nstates == 1 << 22
     ref = Oracle(p).lanczos(3)
