@@ -655,6 +655,12 @@ static hipError_t enqueue_iteration(nv_ctx *c, int pro, int check) {
     hipError_t e = nv::launch_set_iter(c->d_iter, 0, 1, c->stream);
     if (e == hipSuccess) e = nv::launch_normalize(c->dev, c->opts.prec, c->st, c->stream);
     if (e == hipSuccess) e = launch_hv_iter_any(c);
+    if (pro && !check && nv::pro_fusable(c->pro)) {
+        // beta, the omega recurrence and the range decision in one workgroup
+        if (e == hipSuccess) e = nv::launch_lanczos_step_only(c->dev, c->opts.prec, c->st, c->stream);
+        if (e == hipSuccess) e = nv::pro_step_fused(c->pro, c->dev, c->st, nv::hv_blocks(c->dev), c->stream);
+        return e;
+    }
     if (e == hipSuccess) e = nv::launch_lanczos_step(c->dev, c->opts.prec, c->st, c->stream);
     if (pro) {
         if (e == hipSuccess) e = nv::pro_set_omega(c->pro, c->dev, c->st, check, c->stream);

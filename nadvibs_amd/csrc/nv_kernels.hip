@@ -946,6 +946,10 @@ hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStr
     return launch_finish_sum(st.partials, hv_blocks(op), st.alpha, st.iter, 0, s);
 }
 
+hipError_t launch_lanczos_step_only(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    return step_any(op, prec, st, s);
+}
+
 hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     hipError_t e = step_any(op, prec, st, s);
     if (e != hipSuccess) return e;

@@ -35,12 +35,12 @@ __global__ void pro_init_kernel(ProDev p) {
     *p.njl = 0;
 }
 
-// omega recurrence (Source/top.f90:862-874) and the row swap (:875-879).
-__global__ __launch_bounds__(kBlock) void pro_recur_kernel(ProDev p, LzState st) {
+// omega recurrence (Source/top.f90:862-874) and the row swap (:875-879), one
+// workgroup; bi = beta(it).
+__device__ void recur_body(const ProDev &p, const LzState &st, double bi) {
     const int64_t it = *st.iter;
     double *o1 = p.omega, *o2 = p.omega + p.ostride;
     const uint64_t base = (uint64_t)p.meta[2];
-    const double bi = st.beta[it];
     if (threadIdx.x == 0) {
         if (it < p.niter) o2[it + 1] = 1.0;
         o1[it] = p.noise * noise_gauss(p.seed, base, 0.6) * p.epsn * st.beta[1] / bi;
@@ -58,6 +58,10 @@ __global__ __launch_bounds__(kBlock) void pro_recur_kernel(ProDev p, LzState st)
     }
     __syncthreads();
     if (threadIdx.x == 0) p.meta[2] = (int64_t)base + it;
+}
+
+__global__ __launch_bounds__(kBlock) void pro_recur_kernel(ProDev p, LzState st) {
+    recur_body(p, st, st.beta[*st.iter]);
 }
 
 // Dots of a and b (either may be null) with the Krylov vectors q_j listed in
@@ -302,7 +306,7 @@ __device__ int64_t scan_excl(int64_t v, int64_t *total) {
     return before + x - v;
 }
 
-__global__ __launch_bounds__(kDecide) void pro_decide_kernel(ProDev p, LzState st) {
+__device__ void decide_body(const ProDev &p, const LzState &st) {
     __shared__ int64_t sh_nr, sh_n, sh_first;
     const int64_t it = *st.iter;
     double *o2 = p.omega + p.ostride;
@@ -435,6 +439,8 @@ __global__ __launch_bounds__(kDecide) void pro_decide_kernel(ProDev p, LzState s
     }
 }
 
+__global__ __launch_bounds__(kDecide) void pro_decide_kernel(ProDev p, LzState st) { decide_body(p, st); }
+
 __device__ double pro_block_sum(double v) {
     __shared__ double red[kBlock / 64];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -485,6 +491,33 @@ __global__ __launch_bounds__(kBlock) void pro_beta_kernel(ProDev p, LzState st, 
     for (int i = threadIdx.x; i < nparts; i += kBlock) s += st.partials[i];
     s = pro_block_sum(s);
     if (threadIdx.x == 0) st.beta[*st.iter] = sqrt(s);
+}
+
+// An iteration without an exact check in one workgroup: beta(it) = |r| from
+// lanczos_step's block partials (summed as finish_sum_kernel sums them: a
+// 256-thread strided pass, then four wave sums in order), the omega
+// recurrence and the range decision -- three launches of the per-call path.
+__global__ __launch_bounds__(kDecide) void pro_step_kernel(ProDev p, LzState st, int nparts) {
+    __shared__ double red[kBlock / 64];
+    __shared__ double sh_beta;
+    const int64_t it = *st.iter;
+    double v = 0.0;
+    if (threadIdx.x < kBlock)
+        for (int i = threadIdx.x; i < nparts; i += kBlock) v += st.partials[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (threadIdx.x < kBlock && (threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < kBlock / 64; ++i) t += red[i];
+        const double b = sqrt(t);
+        st.beta[it] = b;
+        sh_beta = b;
+    }
+    __syncthreads();
+    recur_body(p, st, sh_beta);
+    __syncthreads();
+    decide_body(p, st);
 }
 
 // Mixed modes with the default (FP64) roundoff model: FP32 rounding of H*v
@@ -612,9 +645,9 @@ hipError_t pro_set_omega(ProState &ps, const NvOpDev &dev, const LzState &st, in
     return e;
 }
 
-hipError_t pro_partial_orthog(ProState &ps, const NvOpDev &dev, const LzState &st, hipStream_t s) {
+// The dots and the update of the decided list, and beta after it.
+static hipError_t orthog_apply(const ProState &ps, const NvOpDev &dev, const LzState &st, hipStream_t s) {
     const ProDev &p = ps.d;
-    hipLaunchKernelGGL(pro_decide_kernel, dim3(1), dim3(kDecide), 0, s, p, st);
     hipError_t e = multidot(ps, dev.N, st, p.jlist, p.njl, 0, 0, 1, s);  // count = list length
     if (e != hipSuccess) return e;
     const int64_t N = dev.N;
@@ -625,6 +658,22 @@ hipError_t pro_partial_orthog(ProState &ps, const NvOpDev &dev, const LzState &s
         hipLaunchKernelGGL(pro_axpy_kernel<double>, dim3(g), dim3(kBlock), 0, s, N, p, st);
     hipLaunchKernelGGL(pro_beta_kernel, dim3(1), dim3(kBlock), 0, s, p, st, g);
     return hipGetLastError();
+}
+
+hipError_t pro_partial_orthog(ProState &ps, const NvOpDev &dev, const LzState &st, hipStream_t s) {
+    hipLaunchKernelGGL(pro_decide_kernel, dim3(1), dim3(kDecide), 0, s, ps.d, st);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return orthog_apply(ps, dev, st, s);
+}
+
+bool pro_fusable(const ProState &ps) { return !ps.d.exact; }
+
+hipError_t pro_step_fused(ProState &ps, const NvOpDev &dev, const LzState &st, int nparts, hipStream_t s) {
+    hipLaunchKernelGGL(pro_step_kernel, dim3(1), dim3(kDecide), 0, s, ps.d, st, nparts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return orthog_apply(ps, dev, st, s);
 }
 
 // ints: [0] n, [1] cap, [2] jcap, [3] njl, [4..11] meta, then lo[cap], ro[cap], jlist[jcap];

@@ -73,6 +73,12 @@ int64_t pro_check_period(const ProState &p);
 // check != 0: also the exact overlap check of this iteration.
 hipError_t pro_set_omega(ProState &p, const NvOpDev &dev, const LzState &st, int check, hipStream_t s);
 hipError_t pro_partial_orthog(ProState &p, const NvOpDev &dev, const LzState &st, hipStream_t s);
+// Whether an iteration without an exact check may run as lanczos_step's
+// kernel (without its beta finish) followed by pro_step_fused: beta, the omega
+// recurrence and the range decision in one workgroup (nparts: the step's block
+// partials), then the update.  Not in exact mode (reorthog > 1).
+bool pro_fusable(const ProState &p);
+hipError_t pro_step_fused(ProState &p, const NvOpDev &dev, const LzState &st, int nparts, hipStream_t s);
 hipError_t pro_get_omega(const ProState &p, double *omega, int64_t n);
 // The state of partial re-orthogonalisation after iteration n as a flat
 // record (omega rows 0..n+1, ranges, list, counters, noise position), and its
