@@ -446,7 +446,8 @@ def main():
         "hv_gbps": round(achieved, 1),
         "hv_compute": {"flops_per_launch": hv_flops, "achieved_tflops": round(hv_flops / hv_avg_s / 1e12, 2),
                        "peak_tflops_fp32": FP32_PEAK_TFLOPS,
-                       "frac": round(hv_flops / hv_avg_s / 1e12 / FP32_PEAK_TFLOPS, 4)},
+                       "frac": round(hv_flops / hv_avg_s / 1e12 / FP32_PEAK_TFLOPS, 4),
+                       "frac_of_unpacked_fp32": round(hv_flops / hv_avg_s / 1e12 / (FP32_PEAK_TFLOPS / 2), 4)},
         "pro": {"events_per_job": pro32[0], "vectors_touched_per_job": pro32[1]},
     }
     if res64 is not None:
