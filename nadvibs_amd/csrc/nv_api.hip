@@ -272,7 +272,7 @@ int32_t nv_term_counts(const nv_ctx *c, int64_t *out) {
 static const nv::JitKernels *ensure_jit(nv_ctx *c, int32_t prec) {
     if (!c->jit_on || (prec != NV_PREC_F64 && prec != NV_PREC_F32)) return nullptr;
     // in FP64 the shaped generic kernel over the interleaved copy of q_i is the
-    // faster one (cfg2: 49.7 against 58.6 us); the specialised FP64 kernel serves
+    // faster one (three states, N = 1.38M: 49.7 against 58.6 us); the specialised FP64 kernel serves
     // the operators it does not take
     if (prec == NV_PREC_F64 && nv::hv_shaped(c->dev, NV_PREC_F64)) return nullptr;
     nv::JitKernels &k = c->jit[prec];

@@ -799,9 +799,11 @@ static bool shaped_ok(const NvOpDev &op, int64_t el) {
     return ((int64_t)op.D + 2 * kBlock) * el < ((int64_t)1 << 31);
 }
 
-// FP64 reads the state-major q_i, one load per state and step.  Above
-// kSkipMinBytes (cfg3: 708 MB) the step-table kernel, which skips the loads of
-// out-of-basis steps by a branch, is faster there (55.9 vs 65.7 ms per H*v).
+// The Lanczos iteration in FP64 reads the interleaved copy LzState::v64i (one
+// 16-byte load per two states).  The standalone and batched H*v read the
+// state-major vector, one load per state and step, and keep the step-table
+// form above kSkipMinBytes, where its branch over out-of-basis steps wins
+// (N = 88.5M, four states: 55.9 against 65.7 ms per H*v).
 int64_t hv_skip_min_bytes() { return kSkipMinBytes; }
 
 bool hv_shaped(const NvOpDev &op, int prec) {
@@ -838,9 +840,10 @@ static hipError_t hv_full_ns(const NvOpDev &op, int prec, const double *x, doubl
     return hipGetLastError();
 }
 
-// Two rows per thread pays in FP32 on large operators (cfg3, 22M rows: 36.7 ->
-// 29.7 ms per H*v); in FP64 (55.6 -> 62.2 ms) and below a few million rows
-// (cfg2: 53.8 -> 57.4 us) the one-row kernel is faster.
+// Two rows per thread pays for the step-table kernel in FP32 on large
+// operators (four states, 22M rows: 36.7 -> 29.7 ms per H*v); in FP64 (55.6 ->
+// 62.2 ms) and below a few million rows (three states, 0.46M rows: 53.8 ->
+// 57.4 us) the one-row kernel is faster.
 constexpr int kTwoRowMinRows = 1 << 22;
 
 template <int NS>
@@ -849,7 +852,8 @@ static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hip
     if constexpr (NS <= kShapedMaxStates) {
         // FP64 runs shaped when the caller keeps the interleaved copy
         // one row per thread: the unrolled terms give the loads enough independence
-        // (cfg3 FP32: 17.1 ms against 17.8 ms with two rows; FP64 40.4 against 46.8)
+        // (four states, 22M rows: FP32 17.1 ms against 17.8 ms with two rows; FP64
+        // 40.4 against 46.8 ms)
         if (hv_shaped(op, prec) && (prec == 1 || st.v64i)) {
             const unsigned g = (unsigned)hv_blocks(op);
             const int64_t vbytes = (int64_t)op.D * nv_lo_stride(NS) * (prec == 0 ? 8 : 4);
