@@ -275,6 +275,35 @@ def test_pro_matches_oracle_without_noise():
     assert np.max(np.min(np.abs(conv[:, None] - conv0[None, :]), axis=1)) < 1e-9
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_captured_pro_iterations_equal_the_reference_loop(prec):
+    """run() replays captured iterations, where beta's finishing sum, the
+    omega recurrence and the range decision are one kernel; the reference's
+    loop (update_lanczos, Hv, lanczos_step, set_omega, partial_orthog,
+    Source/main.f90:63-76) launches them one by one.  Same bits: alpha, beta,
+    omega and the PRO counts."""
+    p = workload("cfg1")
+    n = 300
+    with Engine(p) as eng:
+        eng.lanczos_init(niter=n, reorthog=1, prec=prec, chkorthog=50)
+        eng.run(1, n)
+        a1, b1, w1 = eng.coefficients(omega=True)
+        s1 = eng.pro_stats()
+        eng.lanczos_init(niter=n, reorthog=1, prec=prec, chkorthog=50)
+        for i in range(1, n + 1):
+            eng.update_lanczos(i)
+            eng.Hv(i)
+            eng.lanczos_step(i)
+            if i > 1:
+                eng.set_omega(i)
+                eng.partial_orthog(i)
+        a2, b2, w2 = eng.coefficients(omega=True)
+        s2 = eng.pro_stats()
+    print(f"PRO events/touched: {s1} {s2}", flush=True)
+    assert s1 == s2 and s1[0] > 0
+    assert np.array_equal(a1, a2) and np.array_equal(b1, b2) and np.array_equal(w1, w2)
+
+
 def test_pro_with_noise_keeps_orthogonality():
     p = workload("cfg1")
     with Engine(p) as eng:
