@@ -785,9 +785,10 @@ static int grid_stride_blocks(int64_t n) {
     return (int)(b < 8192 ? b : 8192);
 }
 
-// Above op.skip_min_bytes of vector (default kSkipMinBytes) the shaped
-// kernels point the loads of out-of-basis steps outside the buffer (no memory
-// traffic) instead of at a wrong row.  FP64 runs shaped up to kSkipMinBytes.
+// Above op.skip_min_bytes of vector (set per context: kSkipMinBytes, or less
+// for operators with many out-of-basis steps) the shaped kernels point the
+// loads of out-of-basis steps outside the buffer (no memory traffic) instead
+// of at a wrong row.
 constexpr int64_t kSkipMinBytes = (int64_t)128 << 20;
 constexpr int kShapedMaxStates = NV_MAXS;
 

@@ -206,6 +206,8 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
                                                {2, 3, 0}, {1, 2, 0}, {2, 1, 0}, {2, 2, 2}};
     std::vector<std::vector<NvTermShape>> by_shape(NV_NSHAPES);
     op.shaped = true;
+    double valid_rows = 0.0;
+    int64_t all_steps = 0;
     for (size_t ti = 0; ti < op.terms.size(); ++ti) {
         Term &t = op.terms[ti];
         std::vector<double> C(ns * ns, 0.0);
@@ -246,6 +248,8 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
                     op.dev_steps.push_back(s);
                     dt.nsteps++;
                     op.nnz_fma += rows * npairs;
+                    valid_rows += (double)rows;
+                    ++all_steps;
                 }
                 int j = t.cnt - 1;
                 while (j >= 0 && d[j] >= t.smax[j]) { d[j] = -t.smax[j]; --j; }
@@ -300,6 +304,7 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
         }
     }
     (void)trm_rows;
+    op.step_valid = all_steps > 0 ? valid_rows / ((double)all_steps * (double)op.D) : 1.0;
     op.dev_shapes.clear();
     for (int q = 0; q < NV_NSHAPES; ++q) {
         op.shape_grp[q] = (int32_t)op.dev_shapes.size();

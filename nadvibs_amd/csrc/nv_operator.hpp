@@ -46,6 +46,7 @@ struct Operator {
     std::vector<double> dconst;          // [ns][ns]
     int32_t ncodes = 0;
     int64_t nnz_fma = 0;                 // FMAs of one H*v (nonzero structure)
+    double step_valid = 1.0;             // share of (row, step) pairs inside the basis
     std::vector<int32_t> term_dev;       // [terms] index into dev_terms, -1 if the term has no Delta != 0 step
     std::vector<int32_t> term_ddev;      // [terms] index into dev_dterms, -1 if it has no Delta = 0 part
     // the off-diagonal terms by step shape (valid when every term has one)
