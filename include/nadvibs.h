@@ -51,7 +51,11 @@ extern "C" {
 #define NV_PREC_BF16 2   /* off-diagonal couplings applied to a BF16 copy of the
                             vector (8-bit mantissa) in FP32 arithmetic, diagonal
                             FP64: 2^-9 relative input rounding, for bandwidth
-                            studies, not for the 1e-3 cm^-1 accuracy target */
+                            studies, not for the 1e-3 cm^-1 accuracy target
+                            (experimental).  nv_lanczos_init refuses it with
+                            reorthog > 0: on a 1.38M-row, 2000-iteration run
+                            the rounding outgrew the re-orthogonalisation and
+                            the coefficients went non-finite at iteration 1570 */
 
 /* ---- problem description ---------------------------------------------- */
 /* The data of basis.in / nadvibs.in after reading, before the reference's
@@ -157,7 +161,9 @@ typedef struct nv_lanczos_opts {
                                  instead of the default (sqrt(eps), or
                                  NV_PRO_MIXED_TAU in the mixed modes) */
     double pro_eta;           /* > 0: range end on |omega| instead of the
-                                 default (eps^(3/4), or NV_PRO_MIXED_ETA) */
+                                 default (eps^(3/4), or NV_PRO_MIXED_ETA);
+                                 nv_lanczos_init refuses eta >= tau after the
+                                 defaults are applied */
 } nv_lanczos_opts;
 
 /* Allocates the Krylov basis for opts->niter iterations, builds the start

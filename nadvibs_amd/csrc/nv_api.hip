@@ -418,6 +418,18 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     if (!valid_prec(o->prec)) return fail(c, NV_ERR_ARG, "unknown precision");
     if (o->reorthog > 0 && nv::pro_decide_lds(o->niter) != 0)
         return fail(c, NV_ERR_ARG, "re-orthogonalisation supports up to 32768 iterations");
+    if (o->reorthog > 0 && o->prec == NV_PREC_BF16)
+        return fail(c, NV_ERR_ARG,
+                    "NV_PREC_BF16 does not support re-orthogonalisation: the 2^-9 input rounding outgrows it "
+                    "(non-finite coefficients on long runs); use NV_PREC_F32");
+    if (o->reorthog > 0) {
+        // the range detection assumes that no index is both small (|omega| < eta) and large (>= tau)
+        double tau = 0.0, eta = 0.0;
+        nv::pro_effective_thresholds(*o, &tau, &eta);
+        if (!(eta < tau))
+            return fail(c, NV_ERR_ARG, "re-orthogonalisation needs pro_eta < pro_tau (after the defaults: tau " +
+                                           std::to_string(tau) + ", eta " + std::to_string(eta) + ")");
+    }
     NV_HIP(c, hipSetDevice(c->device));
     NV_HIP(c, hipStreamSynchronize(c->stream));
     const int64_t N = c->op.N;
@@ -555,6 +567,12 @@ int32_t nv_hv_kernel_info(nv_ctx *c, int32_t prec, char *buf, int64_t len) {
     std::string t = k ? "operator-specialised (hiprtc)"
                       : (std::string(nv::hv_shaped(c->dev, prec) ? "generic shaped" : "generic step-table") +
                          (c->jit_log.empty() ? std::string() : ": " + c->jit_log));
+    if (!k && nv::hv_shaped(c->dev, prec)) {
+        char extra[96];
+        std::snprintf(extra, sizeof extra, " [%s; steps inside the basis %.6f]",
+                      nv::hv_shaped_skip(c->dev, prec) ? "skip form" : "plain form", c->op.step_valid);
+        t += extra;
+    }
     std::snprintf(buf, (size_t)len, "%s", t.c_str());
     return NV_OK;
 }

@@ -813,6 +813,11 @@ bool hv_shaped(const NvOpDev &op, int prec) {
     return false;
 }
 
+bool hv_shaped_skip(const NvOpDev &op, int prec) {
+    const int64_t vbytes = (int64_t)op.D * nv_lo_stride(op.ns) * (prec == 0 ? 8 : 4);
+    return hv_shaped(op, prec) && vbytes > op.skip_min_bytes;
+}
+
 // Standalone and batched H*v read the state-major FP64 vector itself.
 static bool shaped_state_major(const NvOpDev &op) {
     return shaped_ok(op, 8) && (int64_t)op.D * 8 * op.ns <= kSkipMinBytes;
@@ -857,8 +862,7 @@ static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hip
         // 40.4 against 46.8 ms)
         if (hv_shaped(op, prec) && (prec == 1 || st.v64i)) {
             const unsigned g = (unsigned)hv_blocks(op);
-            const int64_t vbytes = (int64_t)op.D * nv_lo_stride(NS) * (prec == 0 ? 8 : 4);
-            if (vbytes > op.skip_min_bytes) {
+            if (hv_shaped_skip(op, prec)) {
                 if (prec == 0)
                     hipLaunchKernelGGL((hv_shaped_iter_kernel<NS, F64I, double, 1, true>), dim3(g), dim3(kBlock), lds,
                                        s, op, st);

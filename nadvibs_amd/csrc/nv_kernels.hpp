@@ -40,6 +40,9 @@ size_t hv_lds_bytes(const NvOpDev &op, int prec);
 // Lanczos iteration then reads the interleaved copy LzState::v64i, which the
 // caller allocates (D * nv_lo_stride(ns) doubles, zeroed).
 bool hv_shaped(const NvOpDev &op, int prec);
+// Whether the shaped iteration H*v of precision prec points the loads of
+// out-of-basis steps outside the vector (vector above op.skip_min_bytes).
+bool hv_shaped_skip(const NvOpDev &op, int prec);
 // The same for the batched H*v (which reads the state-major FP64 vector).
 bool hv_shaped_batch(const NvOpDev &op, int prec);
 // Default of NvOpDev::skip_min_bytes (128 MB).

@@ -542,6 +542,13 @@ static void pro_thresholds(ProDev &p, const nv_lanczos_opts &o) {
     p.eta = o.pro_eta > 0 ? o.pro_eta : (mixed_model(o) ? NV_PRO_MIXED_ETA : pow(p.eps, 0.75));
 }
 
+void pro_effective_thresholds(const nv_lanczos_opts &o, double *tau, double *eta) {
+    ProDev p{};
+    pro_thresholds(p, o);
+    *tau = p.seps;
+    *eta = p.eta;
+}
+
 hipError_t pro_init(ProState &ps, const nv_lanczos_opts &o, const Operator &op, const NvOpDev &dev,
                     const LzState &st, std::vector<void *> &allocs, hipStream_t s) {
     (void)dev; (void)st;

@@ -68,6 +68,10 @@ struct ProState {
 hipError_t pro_init(ProState &p, const nv_lanczos_opts &o, const Operator &op, const NvOpDev &dev,
                     const LzState &st, std::vector<void *> &allocs, hipStream_t s);
 size_t pro_decide_lds(int64_t niter);
+// The trigger tau and range end eta that a run with options o uses, after
+// the defaults (sqrt(eps), eps^(3/4); NV_PRO_MIXED_TAU / _ETA in the mixed
+// modes) are applied.
+void pro_effective_thresholds(const nv_lanczos_opts &o, double *tau, double *eta);
 hipError_t pro_reset(ProState &p, const nv_lanczos_opts &o, hipStream_t s);
 int64_t pro_check_period(const ProState &p);
 // check != 0: also the exact overlap check of this iteration.

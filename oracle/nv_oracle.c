@@ -429,6 +429,153 @@ void or_hv(const OOp *op, const double *x, double *y) {
 }
 
 
+void or_start_vector(const OOp *op, double *q);
+
+/* One row r of y = H x (all states), the sums of or_hv in the same order for
+ * that row: the diagonal part first (harmonic energy from the digits, the
+ * dv block of Source/top.f90:733-754), then terms -> steps -> state pairs
+ * (:758-808).  With the float emulation the off-diagonal part accumulates
+ * in float from a float copy of x, as or_hv does. */
+static void o_row(const OOp *op, const double *x, long r, double *yr) {
+    const int ns = op->ns, M = op->M;
+    const long D = op->D;
+    const int lower = (op->quirks & NV_QUIRK_LOWERDIAG) != 0;
+    long nd[64];
+    double dkin = 0.0;
+    for (int j = 0; j < M; ++j) {
+        nd[j] = (r / op->shft[j]) % op->nfunc[j];
+        dkin += ((double)(nd[j] + 1) - 0.5) * op->omega[j];
+    }
+    double dv[16][16];
+    for (int b = 0; b < op->nstblks; ++b) {
+        dv[op->bi[b]][op->bj[b]] = op->concoef[b];
+        dv[op->bj[b]][op->bi[b]] = op->concoef[b];
+    }
+    for (int t = 0; t < op->nterms; ++t) {
+        const OTerm *T = &op->terms[t];
+        if (!T->diag || T->ndir == 0) continue;
+        double d1 = 1.0;
+        for (int l = 0; l < T->cnt; ++l) d1 = d1 * o_h(op, T->pow[l], nd[T->mode[l]], 0);
+        for (int q = 0; q < T->npairs; ++q) dv[T->prow[q]][T->pcol[q]] += T->pcoef[q] * d1;
+    }
+    for (int s = 0; s < ns; ++s) yr[s] = (dv[s][s] + dkin) * x[s * D + r];
+    for (int q = 0; q < op->ndblks; ++q) {
+        int i = op->dbl_i[q], j = op->dbl_j[q];
+        yr[i] += dv[i][j] * x[j * D + r];
+        if (!lower) yr[j] += dv[j][i] * x[i * D + r];
+    }
+    float yf[16] = {0};
+    for (int t = 0; t < op->nterms; ++t) {
+        const OTerm *T = &op->terms[t];
+        if (T->ndir == 0) continue;
+        int d[MAXF];
+        for (int l = 0; l < T->cnt; ++l) d[l] = -T->smax[l];
+        for (long k = 0; k < T->ndir; ++k) {
+            int zero = 1;
+            long stride = 0;
+            for (int l = 0; l < T->cnt; ++l) { if (d[l]) zero = 0; stride += d[l] * op->shft[T->mode[l]]; }
+            if (!zero) {
+                double d1 = 1.0;
+                int ok = 1;
+                for (int l = 0; l < T->cnt && ok; ++l) {
+                    long n = nd[T->mode[l]];
+                    long n2 = n + d[l];
+                    if (n2 < 0 || n2 >= op->nfunc[T->mode[l]]) { ok = 0; break; }
+                    d1 = d1 * o_h(op, T->pow[l], n < n2 ? n : n2, abs(d[l]));
+                }
+                if (ok) {
+                    if (g_off32) {
+                        for (int q = 0; q < T->npairs; ++q)
+                            yf[T->prow[q]] += (float)x[T->pcol[q] * D + r + stride] * (float)T->pcoef[q] * (float)d1;
+                    } else {
+                        for (int q = 0; q < T->npairs; ++q)
+                            yr[T->prow[q]] += x[T->pcol[q] * D + r + stride] * T->pcoef[q] * d1;
+                    }
+                }
+            }
+            int l = T->cnt - 1;
+            while (l >= 0 && d[l] >= T->smax[l]) { d[l] = -T->smax[l]; --l; }
+            if (l < 0) break;
+            d[l] += 2;
+        }
+    }
+    if (g_off32)
+        for (int s = 0; s < ns; ++s) yr[s] += (double)yf[s];
+}
+
+/* y at the listed rows: y[i*ns + s] = (H x)[s*D + rows[i]].  Agrees with
+ * or_hv to the rounding of the harmonic energy (or_hv sums it by the
+ * reference's odometer). */
+void or_hv_rows(const OOp *op, const double *x, long nrows, const long *rows, double *y) {
+    for (long i = 0; i < nrows; ++i) o_row(op, x, rows[i], y + i * op->ns);
+}
+
+/* y = H x with nthreads OpenMP threads over contiguous blocks of rows (the
+ * reference's Global-Arrays row partition, Source/top.f90:400-418). */
+void or_hv_par(const OOp *op, const double *x, double *y, int nthreads) {
+    const long D = op->D;
+    const int ns = op->ns;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long r = 0; r < D; ++r) {
+        double yr[16];
+        o_row(op, x, r, yr);
+        for (int s = 0; s < ns; ++s) y[s * D + r] = yr[s];
+    }
+}
+
+/* y = H x for the rows [r0, r1) only (other rows untouched), nthreads
+ * threads: a bounded sample of or_hv_par. */
+void or_hv_par_range(const OOp *op, const double *x, double *y, long r0, long r1, int nthreads) {
+    const long D = op->D;
+    const int ns = op->ns;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long r = r0; r < r1; ++r) {
+        double yr[16];
+        o_row(op, x, r, yr);
+        for (int s = 0; s < ns; ++s) y[s * D + r] = yr[s];
+    }
+}
+
+/* Iterations i0..i1 of the plain recurrence (reorthog = 0) with nthreads
+ * threads, in the reference's order (Source/top.f90:646-670, :820-842):
+ * state (q1, q2, q3) of 3N doubles; at i0 = 1 the default or FC start
+ * vector is built into q3.  alpha/beta as or_lanczos (alpha[i-1], beta[i]). */
+void or_lanczos_par(const OOp *op, long i0, long i1, int nthreads, double *state, double *alpha, double *beta) {
+    const long N = op->N;
+    double *q1 = state, *q2 = state + N, *q3 = state + 2 * N;
+    if (i0 == 1) {
+        memset(state, 0, (size_t)3 * N * sizeof(double));
+        or_start_vector(op, q3);
+        double b0 = 0.0;
+#pragma omp parallel for num_threads(nthreads) reduction(+ : b0) schedule(static)
+        for (long i = 0; i < N; ++i) b0 += q3[i] * q3[i];
+        beta[0] = sqrt(b0);
+    }
+    for (long it = i0; it <= i1; ++it) {
+        const double sc = 1.0 / beta[it - 1];
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (long i = 0; i < N; ++i) {
+            q1[i] = q2[i];
+            q2[i] = q3[i] * sc;
+        }
+        or_hv_par(op, q2, q3, nthreads);
+        double a = 0.0;
+#pragma omp parallel for num_threads(nthreads) reduction(+ : a) schedule(static)
+        for (long i = 0; i < N; ++i) a += q2[i] * q3[i];
+        alpha[it - 1] = a;
+        const double b = it > 1 ? beta[it - 1] : 0.0;
+        double nb = 0.0;
+#pragma omp parallel for num_threads(nthreads) reduction(+ : nb) schedule(static)
+        for (long i = 0; i < N; ++i) {
+            double v = q3[i] - a * q2[i];
+            if (it > 1) v = v - b * q1[i];
+            q3[i] = v;
+            nb += v * v;
+        }
+        beta[it] = sqrt(nb);
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Franck-Condon start vector (Source/secondary.f90:351-783)            */
 /* ------------------------------------------------------------------ */

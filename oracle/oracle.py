@@ -52,6 +52,10 @@ def lib():
         l.or_lanczos.restype = ctypes.c_int
         l.or_lanczos.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_int, ctypes.c_long, ctypes.c_uint64,
                                  ctypes.c_double, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long)]
+        l.or_hv_rows.argtypes = [ctypes.c_void_p, _dp, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _dp]
+        l.or_hv_par.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
+        l.or_hv_par_range.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_long, ctypes.c_long, ctypes.c_int]
+        l.or_lanczos_par.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_long, ctypes.c_int, _dp, _dp, _dp]
         l.or_set_off32.argtypes = [ctypes.c_int]
         l.or_set_eps_pro.argtypes = [ctypes.c_double]
         l.or_set_pro_thresholds.argtypes = [ctypes.c_double, ctypes.c_double]
@@ -106,6 +110,36 @@ class Oracle:
         y = np.empty_like(x)
         lib().or_hv(self.h, _d(x), _d(y))
         return y
+
+    def hv_rows(self, x, rows):
+        """(H x) at the given rows, all states: out[i, s] = (H x)[s*dimen + rows[i]]."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        assert x.size == self.n and rows.min() >= 0 and rows.max() < self.dimen
+        y = np.empty((rows.size, self.prob.nstates))
+        lib().or_hv_rows(self.h, _d(x), rows.size, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _d(y))
+        return y
+
+    def hv_par(self, x, threads, rows=None):
+        """H x with OpenMP over contiguous row blocks; rows=(r0, r1) computes
+        only those rows (the rest of the result is zero)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        if rows is None:
+            lib().or_hv_par(self.h, _d(x), _d(y), int(threads))
+        else:
+            lib().or_hv_par_range(self.h, _d(x), _d(y), int(rows[0]), int(rows[1]), int(threads))
+        return y
+
+    def lanczos_par(self, niter, threads, state=None, alpha=None, beta=None, i0=1):
+        """Iterations i0..niter of the plain recurrence (reorthog 0) with
+        OpenMP threads; pass the returned (state, alpha, beta) back to go on."""
+        if state is None:
+            state = np.zeros(3 * self.n)
+            alpha = np.zeros(niter)
+            beta = np.zeros(niter + 1)
+        lib().or_lanczos_par(self.h, int(i0), int(niter), int(threads), _d(state), _d(alpha), _d(beta))
+        return state, alpha, beta
 
     def start_vector(self):
         q = np.empty(self.n)

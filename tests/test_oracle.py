@@ -232,3 +232,56 @@ def test_mixed_mode_needs_frequent_exact_checks():
     j = np.argmin(np.abs(ev32[None, :] - ev64[conv][:, None]), axis=1)
     assert np.max(np.abs(ev32[j] - ev64[conv])) * 219474.6313705 < 1e-3
     assert np.max(np.abs(it32[j] - it64[conv])) < 1e-6
+
+
+@pytest.mark.parametrize("quirks", [3, 0])
+@pytest.mark.parametrize("case", SWEEP[:6], ids=[f"s{i}" for i in range(6)])
+def test_hv_rows_matches_dense_and_full(case, quirks):
+    """The row-sampled H*v (or_hv_rows, the checker of the full-size GPU
+    H*v) equals the dense brute force and the whole-vector restatement at
+    every row; in the float emulation it equals or_hv's float sums."""
+    from oracle.oracle import set_off32
+    p = _prob(case, quirks)
+    H = dense_hamiltonian(p)
+    o = Oracle(p)
+    x = np.random.default_rng(7).standard_normal(p.n)
+    ref = H @ x
+    rows = np.arange(o.dimen)
+    got = o.hv_rows(x, rows).T.reshape(-1)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert np.max(np.abs(got - o.hv(x))) <= 1e-14 * np.max(np.abs(ref))
+    set_off32(True)
+    try:
+        full32 = o.hv(x)
+        rows32 = o.hv_rows(x, rows).T.reshape(-1)
+    finally:
+        set_off32(False)
+    assert np.max(np.abs(rows32 - full32)) <= 1e-14 * np.max(np.abs(ref))
+    # a shuffled subset gives the same rows
+    sub = np.random.default_rng(1).permutation(o.dimen)[: max(1, o.dimen // 3)]
+    np.testing.assert_array_equal(o.hv_rows(x, sub), o.hv_rows(x, rows)[sub])
+
+
+def test_parallel_hv_and_recurrence_match_serial():
+    """The OpenMP restatement (the multi-core CPU baseline) against the
+    serial one: H*v and the plain recurrence's coefficients."""
+    p = workload("cfg1")
+    o = Oracle(p)
+    x = np.random.default_rng(3).standard_normal(o.n)
+    y = o.hv(x)
+    for threads in (1, 3):
+        assert np.max(np.abs(o.hv_par(x, threads) - y)) <= 1e-13 * np.max(np.abs(y))
+    part = o.hv_par(x, 2, rows=(100, 500))
+    full = o.hv_par(x, 2)
+    idx = np.zeros(o.dimen, bool)
+    idx[100:500] = True
+    mask = np.tile(idx, p.nstates)
+    np.testing.assert_array_equal(part[mask], full[mask])
+    assert not np.any(part[~mask])
+    r = o.lanczos(40)
+    st, a, b = o.lanczos_par(20, 4)
+    a2, b2 = np.zeros(40), np.zeros(41)
+    a2[:20], b2[:21] = a, b
+    st, a2, b2 = o.lanczos_par(40, 2, st, a2, b2, i0=21)
+    assert np.max(np.abs(a2 - r["alpha"]) / np.abs(r["alpha"])) < 1e-11
+    assert np.max(np.abs(b2 - r["beta"]) / np.abs(r["beta"])) < 1e-11
