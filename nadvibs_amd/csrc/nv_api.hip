@@ -54,6 +54,9 @@ struct nv_ctx {
     nv::JitKernels jit[2];
     std::string jit_log;
     float *hx32 = nullptr;
+    double *hx64i = nullptr;              // interleaved FP64 copy of a standalone H*v input (two-pass form)
+    void *z2 = nullptr;                   // channel scratch of the two-pass H*v
+    size_t z2_bytes = 0;
 
     // raw nadvibs.in coefficients (for the frequency chain rule of the adjoint)
     std::vector<double> potraw;
@@ -177,6 +180,39 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
     if (e == hipSuccess && op.shaped) e = upload(c, op.dev_shapes, &d.tshape, c->allocs);
     for (int q = 0; q <= NV_NSHAPES; ++q) d.tgrp[q] = d.tshape ? op.shape_grp[q] : 0;
     c->tshape = d.tshape;
+    // Two-pass H*v: by default in FP64 from 128 MB of vector (the 88.5M-row cubic operator:
+    // 37.9 against 40.5 ms per H*v); in FP32 the one-pass shaped kernel stays ahead there
+    // (17.05 against 18.1 ms), so FP32 takes it only with NV_HV_TWOPASS=1, which also
+    // drops the size threshold; NV_HV_TWOPASS=0 turns it off.
+    d.hv2 = NvHv2Dev{};
+    d.hv2_min_bytes = (int64_t)128 << 20;
+    {
+        const char *k = std::getenv("NV_HV_TWOPASS");
+        const bool force = k && std::strcmp(k, "1") == 0;
+        if (force) d.hv2_min_bytes = 0;
+        d.hv2.f32 = force ? 1 : 0;
+        if (op.hv2 && !(k && std::strcmp(k, "0") == 0)) {
+            NvHv2Dev &h = d.hv2;
+            h.nl = (int32_t)(op.nfunc[0] * op.nfunc[1]);
+            h.n1 = (int32_t)op.nfunc[1];
+            h.dr = (int32_t)(op.D / h.nl);
+            h.c = std::max(1, nv::kBlock / h.nl);
+            h.gmask = op.hv2_gmask;
+            // FP32 coupling products on v_mfma_f32_4x4x1 measured slower than the VALU
+            // (20.3 against 18.1 ms on the 88.5M-row operator): opt-in with NV_HV2_MFMA=1
+            const char *m = std::getenv("NV_HV2_MFMA");
+            h.mfma = m && std::strcmp(m, "1") == 0;
+            const char *sk = std::getenv("NV_HV2_SKIP");
+            h.skip = sk ? (std::strcmp(sk, "1") == 0 ? 1 : 0) : -1;
+            for (int q = 0; q <= NV_HV2_NSHAPES; ++q) h.pgrp[q] = op.hv2_pgrp[q];
+            for (int g = 0; g < NV_HV2_NG; ++g) {
+                h.sig[g] = op.hv2_sig[g];
+                h.idcm[g] = op.hv2_idcm[g];
+            }
+            if (e == hipSuccess) e = upload(c, op.hv2_phi, &h.phi, c->allocs);
+            h.on = e == hipSuccess ? 1 : 0;
+        }
+    }
     {
         const char *k = std::getenv("NV_HV_KERNEL");
         if (k && std::strcmp(k, "steps") == 0) d.tshape = nullptr;
@@ -242,6 +278,8 @@ int32_t nv_destroy(nv_ctx *c) {
     for (auto &k : c->jit) nv::jit_release(k);
     free_list(c->gallocs);
     if (c->hx32) hipFree(c->hx32);
+    if (c->hx64i) hipFree(c->hx64i);
+    if (c->z2) hipFree(c->z2);
     free_list(c->allocs);
     for (auto &p : c->ev) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
     if (c->d_startvals) hipFree(c->d_startvals);
@@ -281,6 +319,8 @@ static const nv::JitKernels *ensure_jit(nv_ctx *c, int32_t prec) {
     // faster one (three states, N = 1.38M: 49.7 against 58.6 us); the specialised FP64 kernel serves
     // the operators it does not take
     if (prec == NV_PREC_F64 && nv::hv_shaped(c->dev, NV_PREC_F64)) return nullptr;
+    // large vectors: the two-pass H*v
+    if (c->dev.tshape && nv::hv2_use(c->dev, prec)) return nullptr;
     nv::JitKernels &k = c->jit[prec];
     if (k.ready) return &k;
     if (k.failed) return nullptr;
@@ -307,8 +347,53 @@ static hipError_t launch_hv_iter_st(nv_ctx *c, const nv::LzState &st) {
 
 static hipError_t launch_hv_iter_any(nv_ctx *c) { return launch_hv_iter_st(c, c->st); }
 
+// The channel scratch of the two-pass H*v, at least `bytes` (a larger
+// buffer replaces the old one; captured iterations name it, so they go).
+static hipError_t ensure_z2(nv_ctx *c, size_t bytes) {
+    if (c->z2 && c->z2_bytes >= bytes) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    release_graphs(c);
+    if (c->z2) hipFree(c->z2);
+    c->z2 = nullptr;
+    c->z2_bytes = 0;
+    e = hipMalloc(&c->z2, bytes);
+    if (e == hipSuccess) c->z2_bytes = bytes;
+    else c->z2 = nullptr;
+    return e;
+}
+
+// Whether the two-pass H*v serves precision prec on this context (the
+// shaped kernels selected and the tables built for a large enough vector).
+static bool use_hv2(const nv_ctx *c, int32_t prec) { return c->dev.tshape && nv::hv2_use(c->dev, prec); }
+
 // y = H x (double in and out) on stream s, couplings in precision prec.
 static hipError_t hv_apply(nv_ctx *c, int32_t prec, const double *x, double *y, hipStream_t s) {
+    if (use_hv2(c, prec)) {
+        const size_t il = (size_t)c->op.D * nv_lo_stride((int)c->op.ns);
+        hipError_t e = ensure_z2(c, nv::hv2_scratch_bytes(c->dev, prec));
+        if (e != hipSuccess) return e;
+        const void *xin = nullptr;
+        if (prec == NV_PREC_F32) {
+            if (!c->hx32) {
+                e = hipMalloc(&c->hx32, il * sizeof(float));
+                if (e == hipSuccess) e = hipMemsetAsync(c->hx32, 0, il * sizeof(float), s);
+                if (e != hipSuccess) return e;
+            }
+            e = nv::launch_cast_f32(x, c->hx32, c->op.N, c->op.D, (int)c->op.ns, s);
+            xin = c->hx32;
+        } else {
+            if (!c->hx64i) {
+                e = hipMalloc(&c->hx64i, il * sizeof(double));
+                if (e == hipSuccess) e = hipMemsetAsync(c->hx64i, 0, il * sizeof(double), s);
+                if (e != hipSuccess) return e;
+            }
+            e = nv::launch_cast_f64i(x, c->hx64i, c->op.N, c->op.D, (int)c->op.ns, s);
+            xin = c->hx64i;
+        }
+        if (e != hipSuccess) return e;
+        return nv::launch_hv2_full(c->dev, prec, x, xin, c->z2, y, s);
+    }
     const nv::JitKernels *k = ensure_jit(c, prec);
     if (!k) return nv::launch_hv_full(c->dev, prec, x, y, s);
     const void *xin = x;
@@ -316,7 +401,7 @@ static hipError_t hv_apply(nv_ctx *c, int32_t prec, const double *x, double *y, 
         if (!c->hx32) {
             const size_t bytes = (size_t)c->op.D * nv_lo_stride((int)c->op.ns) * sizeof(float);
             hipError_t e = hipMalloc(&c->hx32, bytes);
-            if (e == hipSuccess) e = hipMemset(c->hx32, 0, bytes);
+            if (e == hipSuccess) e = hipMemsetAsync(c->hx32, 0, bytes, s);
             if (e != hipSuccess) return e;
         }
         hipError_t e = nv::launch_cast_f32(x, c->hx32, c->op.N, c->op.D, (int)c->op.ns, s);
@@ -457,18 +542,21 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
         if (o->prec != NV_PREC_F64) {
             const size_t lo_bytes = (size_t)c->op.D * nv_lo_stride((int)c->op.ns) * el;
             NV_HIP(c, alloc(lo_bytes, &st.vlo));
-            NV_HIP(c, hipMemset(st.vlo, 0, lo_bytes));
+            NV_HIP(c, hipMemsetAsync(st.vlo, 0, lo_bytes, c->stream));
             // re-orthogonalisation reads the FP32 copy of the archive (half the bytes)
             if (o->reorthog > 0) {
                 st.ldf = N + (N & 1);
                 NV_HIP(c, alloc((size_t)st.nslots * st.ldf * sizeof(float), (void **)&st.Vf));
-                NV_HIP(c, hipMemset(st.Vf, 0, (size_t)st.nslots * st.ldf * sizeof(float)));
+                NV_HIP(c, hipMemsetAsync(st.Vf, 0, (size_t)st.nslots * st.ldf * sizeof(float), c->stream));
             }
         }
         NV_HIP(c, alloc((size_t)N * (o->prec == NV_PREC_F64 ? 8 : 4), &st.y));
         NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.alpha));
         NV_HIP(c, alloc((o->niter + 3) * sizeof(double), (void **)&st.beta));
-        const int64_t nparts = std::max<int64_t>(nv::hv_blocks(c->dev), 8192) + 64;
+        // per-workgroup partial sums: the one-pass H*v and the step kernel write hv_blocks, the
+        // two-pass H*v one per workgroup of its second pass
+        int64_t nparts = std::max<int64_t>(nv::hv_blocks(c->dev), 8192) + 64;
+        if (c->dev.hv2.on) nparts = std::max<int64_t>(nparts, nv::hv2_parts(c->dev) + 64);
         NV_HIP(c, alloc(nparts * sizeof(double), (void **)&st.partials));
         NV_HIP(c, alloc(nv::kNumCounters * sizeof(unsigned), (void **)&st.counters));
         NV_HIP(c, alloc(sizeof(int64_t), (void **)&c->d_iter));
@@ -499,9 +587,14 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
             NV_HIP(c, hipMalloc(&p, bytes));
             c->allocs.push_back(p);
             c->v64i = (double *)p;
-            NV_HIP(c, hipMemset(p, 0, bytes));
+            NV_HIP(c, hipMemsetAsync(p, 0, bytes, c->stream));
         }
         st.v64i = c->v64i;
+    }
+    st.z2 = nullptr;
+    if (use_hv2(c, o->prec) && (o->prec == NV_PREC_F32 || st.v64i)) {
+        NV_HIP(c, ensure_z2(c, nv::hv2_scratch_bytes(c->dev, o->prec)));
+        st.z2 = c->z2;
     }
     // start vector in the slot of iteration 1, beta(0) = |start|
     double *s1 = st.V + (1 % st.nslots) * N;
@@ -567,7 +660,10 @@ int32_t nv_hv_kernel_info(nv_ctx *c, int32_t prec, char *buf, int64_t len) {
     std::string t = k ? "operator-specialised (hiprtc)"
                       : (std::string(nv::hv_shaped(c->dev, prec) ? "generic shaped" : "generic step-table") +
                          (c->jit_log.empty() ? std::string() : ": " + c->jit_log));
-    if (!k && nv::hv_shaped(c->dev, prec)) {
+    if (!k && use_hv2(c, prec))
+        t = std::string("generic two-pass (modes 0 and 1 split off; coupling products on ") +
+            (prec == NV_PREC_F32 && c->dev.hv2.mfma ? "MFMA" : "the VALU") + ")";
+    else if (!k && nv::hv_shaped(c->dev, prec)) {
         char extra[96];
         std::snprintf(extra, sizeof extra, " [%s; steps inside the basis %.6f]",
                       nv::hv_shaped_skip(c->dev, prec) ? "skip form" : "plain form", c->op.step_valid);
@@ -1758,14 +1854,16 @@ int32_t nv_batch_init(nv_batch *b, int64_t niter, int32_t prec) {
     NV_BHIP(b, alloc((size_t)b->Ntot * (prec == NV_PREC_F64 ? 8 : 4), &y));
     if (prec == NV_PREC_F32) {
         NV_BHIP(b, alloc((size_t)b->Ltot * sizeof(float), &vlo));
-        NV_BHIP(b, hipMemset(vlo, 0, (size_t)b->Ltot * sizeof(float)));
+        NV_BHIP(b, hipMemsetAsync(vlo, 0, (size_t)b->Ltot * sizeof(float), b->stream));
     }
     NV_BHIP(b, alloc((size_t)P * ld * sizeof(double), (void **)&al));
     NV_BHIP(b, alloc((size_t)P * ld * sizeof(double), (void **)&be));
     NV_BHIP(b, alloc((size_t)maxtiles * sizeof(double), (void **)&parts));
-    NV_BHIP(b, hipMemset(V, 0, (size_t)3 * b->Ntot * sizeof(double)));
-    NV_BHIP(b, hipMemset(al, 0, (size_t)P * ld * sizeof(double)));
-    NV_BHIP(b, hipMemset(be, 0, (size_t)P * ld * sizeof(double)));
+    NV_BHIP(b, hipMemsetAsync(V, 0, (size_t)3 * b->Ntot * sizeof(double), b->stream));
+    NV_BHIP(b, hipMemsetAsync(al, 0, (size_t)P * ld * sizeof(double), b->stream));
+    NV_BHIP(b, hipMemsetAsync(be, 0, (size_t)P * ld * sizeof(double), b->stream));
+    // the start vectors below are plain copies: the clears must have landed first
+    NV_BHIP(b, hipStreamSynchronize(b->stream));
     // start vectors (each problem's own: default or Franck-Condon) into slot 1, beta(0) = |start|
     std::vector<double> beta0(P * ld, 0.0);
     for (int64_t p = 0; p < P; ++p) {

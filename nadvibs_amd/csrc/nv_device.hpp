@@ -58,6 +58,47 @@ struct NvTermShape {
     int32_t code[3][4];    // ftab rows of Delta_j = -s_j + 2i
 };
 
+// Two-pass H*v of large operators.  The two slowest modes (0 and 1) are
+// split off: a term t = C_t (x) P_sigma(modes 0,1) (x) P_phi(modes >= 2).
+// Pass 1 gathers each distinct phi over the fast modes once per row and mixes
+// it into one channel Z_g per sigma signature g; pass 2 applies P_sigma along
+// modes 0 and 1 to the channels.  Signatures g (powers of modes 0 and 1):
+// 0 = (0,0), 1..3 = (1..3, 0), 4..6 = (0, 1..3), 7 = (1,1), 8 = (2,1), 9 = (1,2).
+#define NV_HV2_NG 10
+// Shapes of phi, in group order: the NV_NSHAPES term shapes, then (1) (a phi
+// whose every mode keeps Delta = 0 only).
+#define NV_HV2_NSHAPES (NV_NSHAPES + 1)
+struct NvPhi {
+    NvTermShape sh;        // steps of phi over its modes with more than one step (sh.cm unused)
+    int32_t nd;            // modes of phi whose only step is Delta = 0 (at most 2):
+    int32_t dmode[2];      //   their factor h(p, n, 0) multiplies every step
+    int32_t dcode[2];
+    int32_t cm[NV_HV2_NG]; // per channel g: coupling matrix (row of cmat) of the
+                           // term (sigma = g, phi), or -1
+};
+struct NvSigma {
+    int32_t s0, s1;        // largest |Delta| along modes 0 and 1 (-1: mode absent)
+    int32_t code0[4], code1[4];  // ftab rows of Delta = -s + 2i
+    int32_t corr_cm;       // g = q_a^2 only: cmat row of the pure q_a^2 term (its
+                           // Delta = 0 part is diagonal and is taken out again), or -1
+    int32_t corr_code;     // ftab row of h(2, n_a, 0)
+};
+struct NvHv2Dev {
+    int32_t on;            // tables valid for this operator
+    int32_t nl, n1;        // nfunc_0 * nfunc_1, nfunc_1
+    int32_t dr;            // rows per (n_0, n_1) plane: D / nl
+    int32_t c;             // pass 2: consecutive plane rows per workgroup
+    int32_t gmask;         // signatures present
+    int32_t mfma;          // FP32: coupling products on the matrix cores (else the VALU)
+    int32_t skip;          // pass 1 points the loads of out-of-basis steps outside the vector:
+                           // 1 always, 0 never, -1 in FP64 only
+    int32_t f32;           // the two-pass form also serves FP32
+    int32_t idcm[NV_HV2_NG];  // phi = 1 (terms on modes 0 and 1 only): cmat row per channel, or -1
+    const NvPhi *phi;      // grouped by shape (NV_HV2_NSHAPES groups)
+    int32_t pgrp[NV_HV2_NSHAPES + 1];
+    NvSigma sig[NV_HV2_NG];
+};
+
 // Everything the H*v kernels read, passed by value.
 struct NvOpDev {
     int32_t ns, M, D, N, nmax, ncodes, nterms, ndterms;
@@ -77,4 +118,6 @@ struct NvOpDev {
     const NvTermShape *tshape;      // [nterms] grouped by shape, term order within a group
     int32_t tgrp[NV_NSHAPES + 1];   // first descriptor of each shape group
     int64_t skip_min_bytes;         // shaped H*v: vector size above which out-of-basis loads are skipped
+    NvHv2Dev hv2;                   // two-pass H*v tables (hv2.on = 0: not available)
+    int64_t hv2_min_bytes;          // vector size from which the two-pass H*v serves the iteration
 };

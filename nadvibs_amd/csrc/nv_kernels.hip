@@ -674,6 +674,394 @@ __global__ __launch_bounds__(kBlock) void hv_shaped_full_kernel(NvOpDev op, cons
     for (int s = 0; s < NS; ++s) y[(size_t)s * op.D + r] = dv[s] + (double)acc[0][s];
 }
 
+// ------------------------------------------------------------------------
+// Two-pass H*v (tables in NvHv2Dev).  A term t = C_t (x) P_sigma (x) P_phi
+// splits into sigma, its powers of the two slowest modes 0 and 1, and phi,
+// its monomial over the other modes.  With x_r the row r of x (all states):
+//   pass 1: Z_g[r] = sum over the terms with sigma = g of C_t (P_phi x)[r],
+//           each distinct phi gathered once per row and fed to every
+//           channel g it pairs with;
+//   pass 2: y[r] = Z_0[r] + sum_{g > 0} sum_{Delta_0, Delta_1}
+//           h_0 h_1 Z_g[r + Delta_0 shft_0 + Delta_1 shft_1].
+// The step Delta = 0 of the whole term is diagonal (it lives in the FP64
+// diagonal part): pass 1 leaves out phi's zero step for g = 0 only, and pass 2
+// takes the pure q_a^2 term's C_t h(2, n_a, 0) x_r out again at Delta_a = 0.
+// Pass 1 does the distinct phi's gathers (cubic 4-state operator on 10 modes:
+// 936 of the one-pass form's 1760 per row) and all the coupling products;
+// pass 2 streams the channels along the two slow modes, a workgroup covering
+// every (n_0, n_1) of a few consecutive rows of the faster modes so that each
+// channel row is fetched from memory once.
+// ------------------------------------------------------------------------
+typedef float nv_mf4 __attribute__((ext_vector_type(4)));
+
+// Channel accumulators of pass 1.  VALU form: one row per lane, NS values per
+// channel.  MFMA form (FP32, NS <= 4): the coupling products run on the
+// matrix cores as v_mfma_f32_4x4x1 (16 blocks of 4 lanes): block b of a wave
+// is its rows 4b..4b+3; A = the lane's own phi value of state s', B = row
+// (lane & 3) of C_t at column s', so lane 4b+j, register i accumulates row
+// 4b+i, state j.  The VALU is left with the gathers.
+template <int NS, typename TAcc, bool MF> struct Hv2Acc {
+    TAcc a[NV_HV2_NG][NS];
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int g = 0; g < NV_HV2_NG; ++g)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) a[g][s] = (TAcc)0;
+    }
+    template <int G> __device__ __forceinline__ void mix(const TAcc *C, const TAcc (&u)[NS]) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            TAcc v = a[G][s];
+#pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2) v += C[s * NS + s2] * u[s2];
+            a[G][s] = v;
+        }
+    }
+};
+template <int NS> struct Hv2Acc<NS, float, true> {
+    nv_mf4 a[NV_HV2_NG];
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int g = 0; g < NV_HV2_NG; ++g) a[g] = nv_mf4{0.f, 0.f, 0.f, 0.f};
+    }
+    // C: the lane's row (lane & 3) of the 4x4-padded coupling matrix
+    template <int G> __device__ __forceinline__ void mix(const float *C, const float (&u)[NS]) {
+        const nv_mf4 c = *reinterpret_cast<const nv_mf4 *>(C);
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) a[G] = __builtin_amdgcn_mfma_f32_4x4x1f32(u[s2], c[s2], a[G], 0, 0, 0);
+    }
+};
+
+// The coupling rows a consumer reads: global C_t (VALU) or the lane's row of
+// the padded copy staged in LDS (MFMA).
+template <int NS, typename TAcc, bool MF>
+__device__ __forceinline__ const TAcc *hv2_cmat(const NvOpDev &op, const float *ldsc, int cm) {
+    if constexpr (MF) return reinterpret_cast<const TAcc *>(ldsc + cm * 16 + (threadIdx.x & 3) * 4);
+    else return cmat_of<TAcc>(op) + (size_t)cm * NS * NS;
+}
+
+// acc[g] += C_g u over the channels: cm[g] is the term's cmat row, or -1
+// (uniform over the wave); ucm0 is fed to channel 0, u to the others.
+template <int NS, typename TAcc, bool MF>
+__device__ __forceinline__ void hv2_feed(const NvOpDev &op, const float *ldsc, Hv2Acc<NS, TAcc, MF> &acc,
+                                         const int32_t (&cm)[NV_HV2_NG], const TAcc (&u0)[NS], const TAcc (&u)[NS]) {
+    if (cm[0] >= 0) acc.template mix<0>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[0]), u0);
+    if (cm[1] >= 0) acc.template mix<1>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[1]), u);
+    if (cm[2] >= 0) acc.template mix<2>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[2]), u);
+    if (cm[3] >= 0) acc.template mix<3>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[3]), u);
+    if (cm[4] >= 0) acc.template mix<4>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[4]), u);
+    if (cm[5] >= 0) acc.template mix<5>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[5]), u);
+    if (cm[6] >= 0) acc.template mix<6>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[6]), u);
+    if (cm[7] >= 0) acc.template mix<7>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[7]), u);
+    if (cm[8] >= 0) acc.template mix<8>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[8]), u);
+    if (cm[9] >= 0) acc.template mix<9>(hv2_cmat<NS, TAcc, MF>(op, ldsc, cm[9]), u);
+}
+
+// The phis of shape group grp: steps gathered as in shaped_group (one row per
+// thread, out-of-basis loads pointed outside the vector), then fed to their
+// consumers.  u leaves out phi's zero step (for channel 0, where the whole
+// step would be diagonal), uf keeps it.
+template <int NS, typename TIn, typename TAcc, bool MF, bool SKIP, int K0, int K1, int K2>
+__device__ __forceinline__ void hv2_phi_group(const NvOpDev &op, const XRes<NS, TIn> &xs, unsigned vo,
+                                              const uint8_t *dig, const TAcc *ftab, const float *ldsc, int tid,
+                                              int grp, const TAcc (&xr)[NS], Hv2Acc<NS, TAcc, MF> &acc) {
+    constexpr int J1 = K1 ? K1 : 1, J2 = K2 ? K2 : 1;
+    constexpr bool zs = (K0 & 1) && (K1 == 0 || (K1 & 1)) && (K2 == 0 || (K2 & 1));
+    const NvHv2Dev &h = op.hv2;
+    const int p1 = h.pgrp[grp + 1];
+    for (int p = h.pgrp[grp]; p < p1; ++p) {
+        const NvPhi &ph = h.phi[p];
+        const NvTermShape d = ph.sh;
+        TAcc f0[K0], f1[J1], f2[J2];
+        const uint8_t *dq = dig + tid;
+        const int n0 = dq[d.mode[0] * kBlock];
+        TAcc fd = (TAcc)1;  // the factors of the modes whose only step is Delta = 0
+        for (int j = 0; j < ph.nd; ++j) fd *= ftab[ph.dcode[j] * op.nmax + dq[ph.dmode[j] * kBlock]];
+#pragma unroll
+        for (int i = 0; i < K0; ++i) f0[i] = ftab[d.code[0][i] * op.nmax + n0] * fd;
+        if constexpr (K1 > 0) {
+            const int n1 = dq[d.mode[1] * kBlock];
+#pragma unroll
+            for (int i = 0; i < K1; ++i) f1[i] = ftab[d.code[1][i] * op.nmax + n1];
+        } else {
+            f1[0] = (TAcc)1;
+        }
+        if constexpr (K2 > 0) {
+            const int n2 = dq[d.mode[2] * kBlock];
+#pragma unroll
+            for (int i = 0; i < K2; ++i) f2[i] = ftab[d.code[2][i] * op.nmax + n2];
+        } else {
+            f2[0] = (TAcc)1;
+        }
+        TAcc u[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) u[s] = (TAcc)0;
+#pragma unroll
+        for (int i0 = 0; i0 < K0; ++i0)
+#pragma unroll
+            for (int i1 = 0; i1 < J1; ++i1) {
+                TAcc h01 = f0[i0];
+                if constexpr (K1 > 0) h01 *= f1[i1];
+#pragma unroll
+                for (int i2 = 0; i2 < J2; ++i2) {
+                    if (zs && 2 * i0 + 1 == K0 && (K1 == 0 || 2 * i1 + 1 == K1) && (K2 == 0 || 2 * i2 + 1 == K2))
+                        continue;
+                    int off = d.base + i0 * d.inc[0];
+                    if constexpr (K1 > 0) off += i1 * d.inc[1];
+                    if constexpr (K2 > 0) off += i2 * d.inc[2];
+                    TAcc hf = h01;
+                    if constexpr (K2 > 0) hf *= f2[i2];
+                    // plain form: an out-of-basis step reads some row (or 0 past the ends) times 0
+                    unsigned o = vo + (unsigned)off * XRes<NS, TIn>::kEl;
+                    if constexpr (SKIP) o = hf != (TAcc)0 ? o : 0x80000000u;
+                    xs.template fma<TAcc>(o, hf, u);
+                }
+            }
+        TAcc uf[NS];
+        if constexpr (zs) {
+            TAcc z = f0[(K0 - 1) / 2];
+            if constexpr (K1 > 0) z *= f1[(K1 - 1) / 2];
+            if constexpr (K2 > 0) z *= f2[(K2 - 1) / 2];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) uf[s] = u[s] + z * xr[s];
+        } else {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) uf[s] = u[s];
+        }
+        hv2_feed<NS, TAcc, MF>(op, ldsc, acc, ph.cm, u, uf);
+    }
+}
+
+__device__ __forceinline__ size_t hv2_tables_bytes(const NvOpDev &op, size_t el) {
+    return (((size_t)op.ncodes * op.nmax * el + 15) & ~(size_t)15) + 2 * (size_t)op.M * kBlock;
+}
+
+// Pass 1: Z_g[r] for every channel, x the interleaved copy of the vector
+// (FP32, or FP64 as F64I).  Z is [NV_HV2_NG][D][nv_lo_stride(NS)].
+template <int NS, typename TIn, typename TAcc, bool MF, bool SKIP>
+__global__ __launch_bounds__(kBlock) void hv2_pass1_kernel(NvOpDev op, const void *__restrict__ xin,
+                                                            TAcc *__restrict__ Z) {
+    uint8_t *dig;
+    const TAcc *ftab = stage_tables<TAcc>(op, dig);
+    float *ldsc = nullptr;
+    if constexpr (MF) {
+        extern __shared__ __align__(16) unsigned char lds_raw[];
+        ldsc = reinterpret_cast<float *>(lds_raw + hv2_tables_bytes(op, sizeof(float)));
+        for (int i = threadIdx.x; i < op.nterms * 16; i += blockDim.x) {
+            const int t = i >> 4, j = (i >> 2) & 3, k = i & 3;
+            ldsc[i] = (j < NS && k < NS) ? op.cmat32[(size_t)t * NS * NS + j * NS + k] : 0.f;
+        }
+    }
+    const int tid = threadIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r = b * kBlock + tid;
+    const bool live = r < op.D;
+    if (live) digits_energy(op, r, dig, tid);
+    else for (int m = 0; m < op.M; ++m) dig[m * kBlock + tid] = 0;
+    __syncthreads();
+    using TX = typename std::conditional<std::is_same<TIn, F64I>::value, double, TIn>::type;
+    const XRes<NS, TIn> xs(reinterpret_cast<const TX *>(xin), op.D);
+    const unsigned vo = (unsigned)r * XRes<NS, TIn>::kEl;
+    TAcc xr[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) xr[s] = (TAcc)0;
+    xs.template fma<TAcc>(vo, (TAcc)1, xr);  // row r itself (0 past the end)
+    Hv2Acc<NS, TAcc, MF> acc;
+    acc.zero();
+    const NvHv2Dev &h = op.hv2;
+    hv2_feed<NS, TAcc, MF>(op, ldsc, acc, h.idcm, xr, xr);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 2, 0, 0>(op, xs, vo, dig, ftab, ldsc, tid, 0, xr, acc);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 3, 0, 0>(op, xs, vo, dig, ftab, ldsc, tid, 1, xr, acc);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 4, 0, 0>(op, xs, vo, dig, ftab, ldsc, tid, 2, xr, acc);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 2, 2, 0>(op, xs, vo, dig, ftab, ldsc, tid, 3, xr, acc);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 3, 2, 0>(op, xs, vo, dig, ftab, ldsc, tid, 4, xr, acc);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 2, 3, 0>(op, xs, vo, dig, ftab, ldsc, tid, 5, xr, acc);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 1, 2, 0>(op, xs, vo, dig, ftab, ldsc, tid, 6, xr, acc);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 2, 1, 0>(op, xs, vo, dig, ftab, ldsc, tid, 7, xr, acc);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 2, 2, 2>(op, xs, vo, dig, ftab, ldsc, tid, 8, xr, acc);
+    hv2_phi_group<NS, TIn, TAcc, MF, SKIP, 1, 0, 0>(op, xs, vo, dig, ftab, ldsc, tid, 9, xr, acc);
+    constexpr int P = nv_lo_stride(NS);
+    if constexpr (MF) {
+        const int rw = b * kBlock + (tid & ~63) + 4 * ((tid & 63) >> 2);
+        const int j = tid & 3;
+#pragma unroll
+        for (int g = 0; g < NV_HV2_NG; ++g) {
+            if (!((h.gmask >> g) & 1)) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (rw + i < op.D && j < NS) Z[((size_t)g * op.D + rw + i) * P + j] = acc.a[g][i];
+        }
+    } else {
+        if (!live) return;
+#pragma unroll
+        for (int g = 0; g < NV_HV2_NG; ++g) {
+            if (!((h.gmask >> g) & 1)) continue;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) Z[((size_t)g * op.D + r) * P + s] = acc.a[g][s];
+        }
+    }
+}
+
+// Pass 2, one signature channel g of powers (P0, P1) on modes 0 and 1: the
+// R2 consecutive rows r0..r0+R2-1 of a thread share (n_0, n_1) and so every
+// factor.  Steps leaving the basis have factor 0 (factor table) and their
+// loads are pointed outside the buffer; no branch.  A signature whose step
+// range was cut by a small basis (s < p) takes the generic loop.
+template <int NS, typename TIn, typename TAcc, int R2, int P0, int P1>
+__device__ __forceinline__ void hv2_sig_rows(const NvOpDev &op, const NvSigma &sg, const XRes<NS, TIn> &zg, int r0,
+                                             int n0, int n1, int nf0, TAcc (&y)[R2][NS]) {
+    const NvHv2Dev &h = op.hv2;
+    const TAcc *ft = ftab_of<TAcc>(op);
+    constexpr unsigned kEl = XRes<NS, TIn>::kEl;
+    if (sg.s0 == (P0 ? P0 : -1) && sg.s1 == (P1 ? P1 : -1)) {
+        constexpr int K0 = P0 ? P0 + 1 : 1, K1 = P1 ? P1 + 1 : 1;
+        TAcc f0[K0], f1[K1];
+#pragma unroll
+        for (int i = 0; i < K0; ++i) f0[i] = P0 ? ft[sg.code0[i] * op.nmax + n0] : (TAcc)1;
+#pragma unroll
+        for (int i = 0; i < K1; ++i) f1[i] = P1 ? ft[sg.code1[i] * op.nmax + n1] : (TAcc)1;
+#pragma unroll
+        for (int i0 = 0; i0 < K0; ++i0)
+#pragma unroll
+            for (int i1 = 0; i1 < K1; ++i1) {
+                const int d0 = P0 ? 2 * i0 - P0 : 0, d1 = P1 ? 2 * i1 - P1 : 0;
+                const TAcc f = f0[i0] * f1[i1];
+                const int off = (d0 * h.n1 + d1) * h.dr;
+#pragma unroll
+                for (int k = 0; k < R2; ++k) {
+                    const unsigned o = f != (TAcc)0 ? (unsigned)(r0 + k + off) * kEl : 0x80000000u;
+                    zg.template fma<TAcc>(o, f, y[k]);
+                }
+            }
+        return;
+    }
+    const int k0 = sg.s0 < 0 ? 1 : sg.s0 + 1, k1 = sg.s1 < 0 ? 1 : sg.s1 + 1;
+    for (int i0 = 0; i0 < k0; ++i0) {
+        const int d0 = sg.s0 < 0 ? 0 : 2 * i0 - sg.s0;
+        if (n0 + d0 < 0 || n0 + d0 >= nf0) continue;
+        const TAcc f0 = sg.s0 < 0 ? (TAcc)1 : ft[sg.code0[i0] * op.nmax + n0];
+        for (int i1 = 0; i1 < k1; ++i1) {
+            const int d1 = sg.s1 < 0 ? 0 : 2 * i1 - sg.s1;
+            if (n1 + d1 < 0 || n1 + d1 >= h.n1) continue;
+            const TAcc f1 = sg.s1 < 0 ? (TAcc)1 : ft[sg.code1[i1] * op.nmax + n1];
+            const int off = (d0 * h.n1 + d1) * h.dr;
+#pragma unroll
+            for (int k = 0; k < R2; ++k) zg.template fma<TAcc>((unsigned)(r0 + k + off) * kEl, f0 * f1, y[k]);
+        }
+    }
+}
+
+template <int NS, typename TIn, typename TAcc, int R2, int G, int P0, int P1>
+__device__ __forceinline__ void hv2_channel(const NvOpDev &op, const TAcc *Z, int r0, int n0, int n1, int nf0,
+                                            TAcc (&y)[R2][NS]) {
+    if (!((op.hv2.gmask >> G) & 1)) return;
+    using TX = typename std::conditional<std::is_same<TIn, F64I>::value, double, TIn>::type;
+    const XRes<NS, TIn> zg(reinterpret_cast<const TX *>(Z) + (size_t)G * op.D * nv_lo_stride(NS), op.D);
+    hv2_sig_rows<NS, TIn, TAcc, R2, P0, P1>(op, op.hv2.sig[G], zg, r0, n0, n1, nf0, y);
+}
+
+// Pass 2: y_off from the channels for R2 consecutive rows per thread; ITER:
+// the iteration's epilogue (y in the accumulation type, state-major; alpha
+// partials with the FP64 diagonal, one per workgroup), else y = diag x + y_off
+// in FP64 (standalone H*v).  Workgroup: the h.nl planes (n_0, n_1) times h.c
+// threads of R2 consecutive plane rows each, so that every channel row comes
+// from memory once and is reused from the caches for its neighbours.
+template <int NS, typename TIn, typename TAcc, bool ITER, int R2>
+__global__ __launch_bounds__(kBlock) void hv2_pass2_kernel(NvOpDev op, LzState st, const void *__restrict__ xin,
+                                                            const TAcc *__restrict__ Z,
+                                                            const double *__restrict__ xfull,
+                                                            double *__restrict__ yfull) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    uint8_t *dig = lds_raw;
+    const NvHv2Dev &h = op.hv2;
+    const int tid = threadIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int lpos = tid / h.c;
+    const int rest0 = (b * h.c + (tid - lpos * h.c)) * R2;
+    const bool inl = lpos < h.nl && rest0 < h.dr;
+    const int r0 = inl ? lpos * h.dr + rest0 : 0;
+    const int n0 = lpos / h.n1, n1 = lpos - (lpos / h.n1) * h.n1;
+    const int nf0 = h.nl / h.n1;
+    using TX = typename std::conditional<std::is_same<TIn, F64I>::value, double, TIn>::type;
+    constexpr unsigned kEl = XRes<NS, TIn>::kEl;
+    TAcc y[R2][NS];
+#pragma unroll
+    for (int k = 0; k < R2; ++k)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) y[k][s] = (TAcc)0;
+    if (inl) {
+        if (h.gmask & 1) {
+            const XRes<NS, TIn> z0(reinterpret_cast<const TX *>(Z), op.D);
+#pragma unroll
+            for (int k = 0; k < R2; ++k)
+                if (rest0 + k < h.dr) z0.template fma<TAcc>((unsigned)(r0 + k) * kEl, (TAcc)1, y[k]);
+        }
+        hv2_channel<NS, TIn, TAcc, R2, 1, 1, 0>(op, Z, r0, n0, n1, nf0, y);
+        hv2_channel<NS, TIn, TAcc, R2, 2, 2, 0>(op, Z, r0, n0, n1, nf0, y);
+        hv2_channel<NS, TIn, TAcc, R2, 3, 3, 0>(op, Z, r0, n0, n1, nf0, y);
+        hv2_channel<NS, TIn, TAcc, R2, 4, 0, 1>(op, Z, r0, n0, n1, nf0, y);
+        hv2_channel<NS, TIn, TAcc, R2, 5, 0, 2>(op, Z, r0, n0, n1, nf0, y);
+        hv2_channel<NS, TIn, TAcc, R2, 6, 0, 3>(op, Z, r0, n0, n1, nf0, y);
+        hv2_channel<NS, TIn, TAcc, R2, 7, 1, 1>(op, Z, r0, n0, n1, nf0, y);
+        hv2_channel<NS, TIn, TAcc, R2, 8, 2, 1>(op, Z, r0, n0, n1, nf0, y);
+        hv2_channel<NS, TIn, TAcc, R2, 9, 1, 2>(op, Z, r0, n0, n1, nf0, y);
+        // the pure q_a^2 terms at Delta_a = 0 are diagonal: take them out again
+        for (int g = 2; g <= 5; g += 3) {
+            const NvSigma &sg = h.sig[g];
+            if (!((h.gmask >> g) & 1) || sg.corr_cm < 0) continue;
+            const TAcc hz = ftab_of<TAcc>(op)[sg.corr_code * op.nmax + (g <= 3 ? n0 : n1)];
+            const TAcc *C = cmat_of<TAcc>(op) + (size_t)sg.corr_cm * NS * NS;
+            const XRes<NS, TIn> xs(reinterpret_cast<const TX *>(xin), op.D);
+#pragma unroll
+            for (int k = 0; k < R2; ++k) {
+                TAcc xr[NS];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) xr[s] = (TAcc)0;
+                xs.template fma<TAcc>((unsigned)(r0 + k) * kEl, (TAcc)1, xr);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    TAcc a = (TAcc)0;
+#pragma unroll
+                    for (int s2 = 0; s2 < NS; ++s2) a += C[s * NS + s2] * xr[s2];
+                    y[k][s] -= hz * a;
+                }
+            }
+        }
+    }
+    double part = 0.0;
+#pragma unroll
+    for (int k = 0; k < R2; ++k) {
+        const bool live = inl && rest0 + k < h.dr;
+        const int r = r0 + k;
+        if (op.offstate && live) digits_energy(op, r, dig, tid);
+        if (!live) continue;
+        double v[NS], dv[NS];
+        if constexpr (ITER) {
+            const int64_t it = *st.iter;
+            const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
+            diag_apply<NS>(op, r, dig, tid, v, dv);
+            TAcc *yo = reinterpret_cast<TAcc *>(st.y);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                yo[(size_t)s * op.D + r] = y[k][s];
+                part += v[s] * (dv[s] + (double)y[k][s]);
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) v[s] = xfull[(size_t)s * op.D + r];
+            diag_apply<NS>(op, r, dig, tid, v, dv);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) yfull[(size_t)s * op.D + r] = dv[s] + (double)y[k][s];
+        }
+    }
+    if constexpr (ITER) {
+        const double bs = block_sum(part);
+        if (tid == 0) st.partials[b] = bs;
+    }
+}
+
 // lanczos_step (Source/top.f90:820-842): r = (H q_i - alpha q_i) - beta(i-1) q_{i-1}
 // stored in slot i+1, beta(i) = |r|.
 template <int NS, typename TAcc>
@@ -898,6 +1286,94 @@ static hipError_t hv_iter_ns(const NvOpDev &op, int prec, const LzState &st, hip
     return hipGetLastError();
 }
 
+// ---- two-pass H*v launchers ----
+static size_t hv2_pass1_lds(const NvOpDev &op, int prec);
+
+bool hv2_use(const NvOpDev &op, int prec) {
+    if (!op.tshape || !op.hv2.on || (prec != 0 && prec != 1) || (prec == 1 && !op.hv2.f32)) return false;
+    const int64_t el = (int64_t)nv_lo_stride(op.ns) * (prec == 0 ? 8 : 4);
+    return (int64_t)op.D * el >= op.hv2_min_bytes && ((int64_t)op.D + 2 * kBlock) * el < ((int64_t)1 << 31) &&
+           hv2_pass1_lds(op, prec) <= 64 * 1024;
+}
+
+size_t hv2_scratch_bytes(const NvOpDev &op, int prec) {
+    return (size_t)NV_HV2_NG * op.D * nv_lo_stride(op.ns) * (prec == 0 ? 8 : 4);
+}
+
+// Consecutive plane rows per thread of the second pass.
+constexpr int kHv2Rows = 4;
+
+int hv2_parts(const NvOpDev &op) {
+    const int per = op.hv2.c * kHv2Rows;
+    return (op.hv2.dr + per - 1) / per;
+}
+
+// The FP32 path keeps the coupling products on the matrix cores up to four states.
+static bool hv2_mfma(const NvOpDev &op, int prec) { return prec == 1 && op.ns <= 4 && op.hv2.mfma; }
+
+static size_t hv2_pass1_lds(const NvOpDev &op, int prec) {
+    return hv_lds_bytes(op, prec) + (hv2_mfma(op, prec) ? (size_t)op.nterms * 16 * sizeof(float) : 0);
+}
+
+template <int NS>
+static hipError_t hv2_ns(const NvOpDev &op, int prec, const void *xin, void *Z, const LzState *st,
+                         const double *xfull, double *yfull, hipStream_t s) {
+    if constexpr (NS > 4) {
+        return hipErrorInvalidValue;  // the tables are built for at most four states
+    } else {
+    const unsigned g1 = (unsigned)hv_blocks(op);
+    const size_t l1 = hv2_pass1_lds(op, prec);
+    // plain form in FP32 (18.1 against 19.2 ms), skip form in FP64 (37.9 against 40.3 ms)
+    const bool skip = op.hv2.skip < 0 ? prec == 0 : op.hv2.skip != 0;
+    if (prec == 1) {
+        if (hv2_mfma(op, prec)) {
+            if (skip)
+                hipLaunchKernelGGL((hv2_pass1_kernel<NS, float, float, true, true>), dim3(g1), dim3(kBlock), l1, s,
+                                   op, xin, (float *)Z);
+            else
+                hipLaunchKernelGGL((hv2_pass1_kernel<NS, float, float, true, false>), dim3(g1), dim3(kBlock), l1, s,
+                                   op, xin, (float *)Z);
+        } else {
+            if (skip)
+                hipLaunchKernelGGL((hv2_pass1_kernel<NS, float, float, false, true>), dim3(g1), dim3(kBlock), l1, s,
+                                   op, xin, (float *)Z);
+            else
+                hipLaunchKernelGGL((hv2_pass1_kernel<NS, float, float, false, false>), dim3(g1), dim3(kBlock), l1,
+                                   s, op, xin, (float *)Z);
+        }
+    } else {
+        if (skip)
+            hipLaunchKernelGGL((hv2_pass1_kernel<NS, F64I, double, false, true>), dim3(g1), dim3(kBlock), l1, s, op,
+                               xin, (double *)Z);
+        else
+            hipLaunchKernelGGL((hv2_pass1_kernel<NS, F64I, double, false, false>), dim3(g1), dim3(kBlock), l1, s,
+                               op, xin, (double *)Z);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const unsigned g2 = (unsigned)hv2_parts(op);
+    const unsigned b2 = (unsigned)(op.hv2.nl * op.hv2.c);
+    const size_t l2 = (size_t)op.M * kBlock;
+    const LzState none{};
+    if (st) {
+        if (prec == 1)
+            hipLaunchKernelGGL((hv2_pass2_kernel<NS, float, float, true, kHv2Rows>), dim3(g2), dim3(b2), l2, s, op, *st, xin,
+                               (const float *)Z, nullptr, nullptr);
+        else
+            hipLaunchKernelGGL((hv2_pass2_kernel<NS, F64I, double, true, kHv2Rows>), dim3(g2), dim3(b2), l2, s, op, *st, xin,
+                               (const double *)Z, nullptr, nullptr);
+    } else {
+        if (prec == 1)
+            hipLaunchKernelGGL((hv2_pass2_kernel<NS, float, float, false, kHv2Rows>), dim3(g2), dim3(b2), l2, s, op, none, xin,
+                               (const float *)Z, xfull, yfull);
+        else
+            hipLaunchKernelGGL((hv2_pass2_kernel<NS, F64I, double, false, kHv2Rows>), dim3(g2), dim3(b2), l2, s, op, none, xin,
+                               (const double *)Z, xfull, yfull);
+    }
+    return hipGetLastError();
+    }
+}
+
 template <int NS>
 static hipError_t step_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     const size_t lds = op.offstate ? (size_t)op.M * kBlock : 0;
@@ -937,8 +1413,27 @@ hipError_t launch_hv_full(const NvOpDev &op, int prec, const double *x, double *
     NV_DISPATCH_NS(hv_full_ns, op, prec, x, y, s)
 }
 
+hipError_t launch_hv2_full(const NvOpDev &op, int prec, const double *x, const void *xin, void *Z, double *y,
+                           hipStream_t s) {
+    NV_DISPATCH_NS(hv2_ns, op, prec, xin, Z, nullptr, x, y, s)
+}
+
+// The iteration H*v runs two-pass when the caller gave the channel scratch
+// (LzState::z2) and, in FP64, the interleaved copy of q_i.
+static bool hv2_iter(const NvOpDev &op, int prec, const LzState &st) {
+    return st.z2 && hv2_use(op, prec) && (prec == 1 || st.v64i);
+}
+
 static hipError_t hv_iter_any(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    if (hv2_iter(op, prec, st)) {
+        const void *xin = prec == 1 ? st.vlo : (const void *)st.v64i;
+        NV_DISPATCH_NS(hv2_ns, op, prec, xin, st.z2, &st, nullptr, nullptr, s)
+    }
     NV_DISPATCH_NS(hv_iter_ns, op, prec, st, s)
+}
+
+int hv_iter_parts(const NvOpDev &op, int prec, const LzState &st) {
+    return hv2_iter(op, prec, st) ? hv2_parts(op) : hv_blocks(op);
 }
 
 static hipError_t step_any(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
@@ -952,7 +1447,7 @@ hipError_t launch_hv_iter_only(const NvOpDev &op, int prec, const LzState &st, h
 hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     hipError_t e = hv_iter_any(op, prec, st, s);
     if (e != hipSuccess) return e;
-    return launch_finish_sum(st.partials, hv_blocks(op), st.alpha, st.iter, 0, s);
+    return launch_finish_sum(st.partials, hv_iter_parts(op, prec, st), st.alpha, st.iter, 0, s);
 }
 
 hipError_t launch_lanczos_step_only(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
@@ -984,6 +1479,20 @@ __global__ __launch_bounds__(kBlock) void cast_f32_kernel(const double *x, float
         const int64_t s = i / D;
         y[(i - s * D) * nsp + s] = (float)x[i];
     }
+}
+
+// y[r * nsp + s] = x[s * D + r] (the interleaved FP64 copy).
+__global__ __launch_bounds__(kBlock) void cast_f64i_kernel(const double *x, double *y, int64_t n, int64_t D, int nsp) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const int64_t s = i / D;
+        y[(i - s * D) * nsp + s] = x[i];
+    }
+}
+
+hipError_t launch_cast_f64i(const double *x, double *y, int64_t n, int64_t D, int ns, hipStream_t s) {
+    hipLaunchKernelGGL(cast_f64i_kernel, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, x, y, n, D,
+                       nv_lo_stride(ns));
+    return hipGetLastError();
 }
 
 hipError_t launch_cast_f32(const double *x, float *y, int64_t n, int64_t D, int ns, hipStream_t s) {

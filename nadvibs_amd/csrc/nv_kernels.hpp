@@ -29,6 +29,7 @@ struct LzState {
     int64_t ldf = 0;             // N rounded up to even (pairs of FP32 elements load as one)
     double *v64i = nullptr;      // FP64 runs of the shaped generic H*v: q_i interleaved
                                  // [D][nv_lo_stride(ns)], padding zero, written by normalize
+    void *z2 = nullptr;          // channel scratch of the two-pass H*v (hv2_scratch_bytes), or null
 };
 
 // counters[] slots
@@ -43,6 +44,18 @@ bool hv_shaped(const NvOpDev &op, int prec);
 // Whether the shaped iteration H*v of precision prec points the loads of
 // out-of-basis steps outside the vector (vector above op.skip_min_bytes).
 bool hv_shaped_skip(const NvOpDev &op, int prec);
+// Two-pass H*v (NvHv2Dev): whether it serves precision prec (tables built,
+// vector at least op.hv2_min_bytes), its channel scratch, and the number of
+// alpha partials the iteration H*v then writes.
+bool hv2_use(const NvOpDev &op, int prec);
+size_t hv2_scratch_bytes(const NvOpDev &op, int prec);
+int hv2_parts(const NvOpDev &op);
+int hv_iter_parts(const NvOpDev &op, int prec, const LzState &st);
+// y = H x (x, y state-major FP64) by the two passes; xin is the interleaved
+// copy of x (FP32 for prec 1, FP64 for prec 0), Z the channel scratch.
+hipError_t launch_hv2_full(const NvOpDev &op, int prec, const double *x, const void *xin, void *Z, double *y,
+                           hipStream_t s);
+hipError_t launch_cast_f64i(const double *x, double *y, int64_t n, int64_t D, int ns, hipStream_t s);
 // The same for the batched H*v (which reads the state-major FP64 vector).
 bool hv_shaped_batch(const NvOpDev &op, int prec);
 // Default of NvOpDev::skip_min_bytes (128 MB).

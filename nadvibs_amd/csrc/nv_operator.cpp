@@ -91,6 +91,134 @@ static int64_t ipow_prod(const std::vector<int64_t> &v) {
     return r;
 }
 
+// Signature channel of the powers (p0, p1) of modes 0 and 1 (NvHv2Dev).
+static int hv2_sig_id(int p0, int p1) {
+    if (p1 == 0) return p0;                 // 0..3
+    if (p0 == 0) return 3 + p1;             // 4..6
+    if (p0 == 1 && p1 == 1) return 7;
+    if (p0 == 2 && p1 == 1) return 8;
+    if (p0 == 1 && p1 == 2) return 9;
+    return -1;
+}
+
+// Tables of the two-pass H*v: every term with off-diagonal steps is split
+// into its part on modes 0 and 1 (sigma) and its part on the other modes
+// (phi).  Each distinct phi gets the shaped descriptor of its steps and, per
+// sigma channel, the coupling matrix of the term it forms with that channel.
+// Needs the shaped form (ordr <= 3), at least three modes, at most four
+// states, nfunc_0 * nfunc_1 <= 256 and a known step shape for every phi.
+template <typename CodeOf>
+static void build_two_pass(Operator &op, CodeOf &code_of) {
+    static const int kShapes[NV_NSHAPES][3] = {{2, 0, 0}, {3, 0, 0}, {4, 0, 0}, {2, 2, 0}, {3, 2, 0},
+                                               {2, 3, 0}, {1, 2, 0}, {2, 1, 0}, {2, 2, 2}};
+    op.hv2 = false;
+    op.hv2_phi.clear();
+    op.hv2_gmask = 0;
+    for (int g = 0; g < NV_HV2_NG; ++g) op.hv2_idcm[g] = -1;
+    for (auto &g : op.hv2_sig) {
+        g.s0 = g.s1 = -1;
+        for (int i = 0; i < 4; ++i) g.code0[i] = g.code1[i] = 0;
+        g.corr_cm = -1;
+        g.corr_code = 0;
+    }
+    if (!op.shaped || op.M < 3 || op.ordr > 3 || op.ns > 4 || op.nfunc[0] * op.nfunc[1] > 256) return;
+    struct Tmp {
+        int shape;
+        NvPhi ph;
+    };
+    std::vector<Tmp> phis;
+    std::map<std::vector<std::pair<int, int>>, int> index;
+    for (size_t ti = 0; ti < op.terms.size(); ++ti) {
+        if (op.term_dev[ti] < 0) continue;
+        const Term &t = op.terms[ti];
+        int p0 = 0, p1 = 0, s0 = -1, s1 = -1;
+        std::vector<int> rs;  // slots of t on modes >= 2, in t's order
+        for (int j = 0; j < t.cnt; ++j) {
+            if (t.mode[j] == 0) { p0 = t.pow[j]; s0 = t.smax[j]; }
+            else if (t.mode[j] == 1) { p1 = t.pow[j]; s1 = t.smax[j]; }
+            else rs.push_back(j);
+        }
+        const int g = hv2_sig_id(p0, p1);
+        if (g < 0 || rs.size() > 3) return;
+        NvSigma &sg = op.hv2_sig[g];
+        sg.s0 = p0 ? s0 : -1;
+        sg.s1 = p1 ? s1 : -1;
+        for (int i = 0; p0 && i <= s0; ++i) sg.code0[i] = code_of(0, p0, -s0 + 2 * i);
+        for (int i = 0; p1 && i <= s1; ++i) sg.code1[i] = code_of(1, p1, -s1 + 2 * i);
+        op.hv2_gmask |= 1 << g;
+        const int cm = op.term_dev[ti];
+        if (rs.empty()) {
+            op.hv2_idcm[g] = cm;
+            if ((p0 == 2 && p1 == 0) || (p0 == 0 && p1 == 2)) {
+                sg.corr_cm = cm;
+                sg.corr_code = code_of(p0 ? 0 : 1, 2, 0);
+            }
+            continue;
+        }
+        std::vector<std::pair<int, int>> key;
+        for (int j : rs) key.push_back({t.mode[j], t.pow[j]});
+        auto it = index.find(key);
+        int pi;
+        if (it == index.end()) {
+            // slots with more than one step form the shape; the others (Delta = 0 only)
+            // become diagonal factors, except that a phi with no stepping slot keeps
+            // its first one as the shape (1)
+            std::vector<int> act, dia;
+            for (int j : rs) (t.smax[j] > 0 ? act : dia).push_back(j);
+            if (act.empty()) {
+                act.push_back(dia.front());
+                dia.erase(dia.begin());
+            }
+            if (dia.size() > 2) return;
+            int k[3] = {0, 0, 0};
+            for (size_t j = 0; j < act.size(); ++j) k[j] = t.smax[act[j]] + 1;
+            int sid = -1;
+            for (int q = 0; q < NV_NSHAPES; ++q)
+                if (k[0] == kShapes[q][0] && k[1] == kShapes[q][1] && k[2] == kShapes[q][2]) sid = q;
+            if (k[0] == 1 && k[1] == 0) sid = NV_NSHAPES;
+            if (sid < 0) return;
+            Tmp tp;
+            tp.shape = sid;
+            for (int q = 0; q < NV_HV2_NG; ++q) tp.ph.cm[q] = -1;
+            tp.ph.nd = (int32_t)dia.size();
+            for (int q = 0; q < 2; ++q) {
+                tp.ph.dmode[q] = q < tp.ph.nd ? t.mode[dia[q]] : 0;
+                tp.ph.dcode[q] = q < tp.ph.nd ? code_of(t.mode[dia[q]], t.pow[dia[q]], 0) : 0;
+            }
+            rs = act;
+            NvTermShape &sh = tp.ph.sh;
+            sh.cm = 0;
+            int64_t base = 0;
+            for (int j = 0; j < 3; ++j) {
+                sh.inc[j] = 0;
+                sh.mode[j] = 0;
+                for (int i = 0; i < 4; ++i) sh.code[j][i] = 0;
+            }
+            for (size_t j = 0; j < rs.size(); ++j) {
+                const int m = t.mode[rs[j]], pw = t.pow[rs[j]], sm = t.smax[rs[j]];
+                base -= (int64_t)sm * op.shft[m];
+                sh.inc[j] = (int32_t)(2 * op.shft[m]);
+                sh.mode[j] = m;
+                for (int i = 0; i <= sm; ++i) sh.code[j][i] = code_of(m, pw, -sm + 2 * i);
+            }
+            sh.base = (int32_t)base;
+            pi = (int)phis.size();
+            index[key] = pi;
+            phis.push_back(tp);
+        } else {
+            pi = it->second;
+        }
+        phis[pi].ph.cm[g] = cm;
+    }
+    for (int q = 0; q < NV_HV2_NSHAPES; ++q) {
+        op.hv2_pgrp[q] = (int32_t)op.hv2_phi.size();
+        for (auto &tp : phis)
+            if (tp.shape == q) op.hv2_phi.push_back(tp.ph);
+    }
+    op.hv2_pgrp[NV_HV2_NSHAPES] = (int32_t)op.hv2_phi.size();
+    op.hv2 = true;
+}
+
 bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
     if (pr.nstates < 1 || pr.nstates > NV_MAXS) { err = "nstates must be in 1..10"; return false; }
     if (pr.nmodes < 1 || pr.nmodes > NV_MAXM) { err = "nmodes must be in 1..64"; return false; }
@@ -315,6 +443,7 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
         op.dev_shapes.clear();
         for (int q = 0; q <= NV_NSHAPES; ++q) op.shape_grp[q] = 0;
     }
+    build_two_pass(op, code_of);
     // constant block matrix, gated by the diagonal-block list (Source/top.f90:528-553)
     op.dconst.assign(ns * ns, 0.0);
     int napplied = 0;

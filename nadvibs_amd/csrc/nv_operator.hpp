@@ -53,6 +53,13 @@ struct Operator {
     bool shaped = false;
     std::vector<NvTermShape> dev_shapes;  // [dev_terms] grouped by shape
     int32_t shape_grp[NV_NSHAPES + 1] = {0};
+    // two-pass H*v tables (modes 0 and 1 split off; see NvHv2Dev)
+    bool hv2 = false;
+    std::vector<NvPhi> hv2_phi;
+    int32_t hv2_pgrp[NV_HV2_NSHAPES + 1] = {0};
+    int32_t hv2_idcm[NV_HV2_NG] = {0};
+    int32_t hv2_gmask = 0;
+    NvSigma hv2_sig[NV_HV2_NG];
 
     // start vector data
     int32_t neworigin = 0;

@@ -585,9 +585,9 @@ hipError_t pro_init(ProState &ps, const nv_lanczos_opts &o, const Operator &op, 
     if (e == hipSuccess) e = alloc((size_t)2 * (o.niter + 4) * sizeof(int32_t), (void **)&p.scr);
     if (e == hipSuccess) e = alloc((size_t)p.jcap * p.nchunks * 2 * sizeof(double), (void **)&p.dpart);
     if (e != hipSuccess) return e;
-    e = hipMemsetAsync(p.omega, 0, 2 * p.ostride * sizeof(double), s);
-    if (e == hipSuccess) e = hipMemsetAsync(p.lo, 0, p.cap * sizeof(int64_t), s);
-    if (e == hipSuccess) e = hipMemsetAsync(p.ro, 0, p.cap * sizeof(int64_t), s);
+    e = hipMemset(p.omega, 0, 2 * p.ostride * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(p.lo, 0, p.cap * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemset(p.ro, 0, p.cap * sizeof(int64_t));
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(pro_init_kernel, dim3(1), dim3(64), 0, s, p);
     return hipGetLastError();
@@ -600,9 +600,9 @@ hipError_t pro_reset(ProState &ps, const nv_lanczos_opts &o, hipStream_t s) {
     p.noise = o.noise_scale;
     pro_thresholds(p, o);
     p.epsn = p.eps * (double)p.epsn_scale;
-    hipError_t e = hipMemsetAsync(p.omega, 0, 2 * p.ostride * sizeof(double), s);
-    if (e == hipSuccess) e = hipMemsetAsync(p.lo, 0, p.cap * sizeof(int64_t), s);
-    if (e == hipSuccess) e = hipMemsetAsync(p.ro, 0, p.cap * sizeof(int64_t), s);
+    hipError_t e = hipMemset(p.omega, 0, 2 * p.ostride * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(p.lo, 0, p.cap * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemset(p.ro, 0, p.cap * sizeof(int64_t));
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(pro_init_kernel, dim3(1), dim3(64), 0, s, p);
     return hipGetLastError();

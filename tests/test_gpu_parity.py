@@ -412,3 +412,56 @@ def test_mixed_mode_long_run_against_fp64():
     print(f"max dE vs FP64: {dE:.2e} cm^-1")
     assert dE < 1e-3
     assert np.max(np.abs(it[j] - it64[conv])) < 1e-6
+
+
+TWO_PASS = [
+    # ns, nfunc, ordr, cubic, offdiag_even, quirks: operators of at most four states and three
+    # or more modes, where the two-pass form applies (forced on at any size)
+    (4, [4, 3, 3, 3, 2], 3, 10.0, 0.0, 3),
+    (3, [3, 4, 2, 3], 3, 10.0, 30.0, 3),
+    (2, [5, 2, 3], 2, 0.0, 0.0, 3),
+    (4, [3, 3, 2, 2, 2, 2], 3, 5.0, 20.0, 0),
+    (1, [4, 3, 3], 3, 10.0, 0.0, 3),
+    (3, [1, 3, 3, 3], 3, 10.0, 0.0, 3),
+    (4, [3, 1, 3, 2], 3, 10.0, 5.0, 3),
+]
+
+
+@pytest.mark.parametrize("variant", ["default", "mfma-skip", "valu-plain"])
+@pytest.mark.parametrize("case", TWO_PASS, ids=[f"t{i}" for i in range(len(TWO_PASS))])
+def test_two_pass_hv_matches_oracle(case, variant, monkeypatch):
+    """The two-pass H*v (modes 0 and 1 split off) forced on small operators,
+    in each form (FP32 coupling products on the VALU or on MFMA; pass 1 with
+    out-of-basis loads skipped or plain): standalone H*v in both precisions
+    and the first Lanczos coefficients against the oracle."""
+    monkeypatch.setenv("NV_HV_TWOPASS", "1")
+    monkeypatch.delenv("NV_HV_KERNEL", raising=False)
+    monkeypatch.delenv("NV_HV2_MFMA", raising=False)
+    monkeypatch.delenv("NV_HV2_SKIP", raising=False)
+    if variant == "mfma-skip":
+        monkeypatch.setenv("NV_HV2_MFMA", "1")
+        monkeypatch.setenv("NV_HV2_SKIP", "1")
+    elif variant == "valu-plain":
+        monkeypatch.setenv("NV_HV2_MFMA", "0")
+        monkeypatch.setenv("NV_HV2_SKIP", "0")
+    ns, nf, ordr, cub, offe, quirks = case
+    p = make_problem(ns, nf, ordr, (400, 3000), 250.0, 60.0, 40.0, cubic=cub, seed=5, quirks=quirks,
+                     offdiag_even=offe)
+    orc = Oracle(p)
+    x = np.random.default_rng(3).standard_normal(p.n)
+    ref = orc.hv(x)
+    r = orc.lanczos(12)
+    with Engine(p) as eng:
+        for prec, tol_hv, tol_ab in (("f64", 1e-13, 1e-11), ("f32", 2e-6, 5e-6)):
+            info = eng.hv_kernel(prec)
+            print(f"{prec}: {info}", flush=True)
+            assert info.startswith("generic two-pass"), info
+            if prec == "f32":
+                assert ("MFMA" in info) == (variant == "mfma-skip"), info
+            err = np.max(np.abs(eng.hv(x, prec) - ref)) / np.max(np.abs(ref))
+            assert err <= tol_hv, (prec, err)
+            eng.lanczos_init(niter=12, reorthog=0, prec=prec)
+            eng.run(1, 12)
+            a, b, _ = eng.coefficients()
+            assert np.max(np.abs(a - r["alpha"]) / np.abs(r["alpha"])) <= tol_ab, prec
+            assert np.max(np.abs(b - r["beta"]) / np.abs(r["beta"])) <= tol_ab, prec
