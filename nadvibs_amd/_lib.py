@@ -91,6 +91,9 @@ SIGNATURES = [
                                            _dp, _dp, _i32p, _i64p, _i32p, _i32p, _dp, _dp, _dp, _dp]),
     ("nv_convolve", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, _dp, _dp, ctypes.c_int64, _dp,
                                      ctypes.c_double, _dp]),
+    ("nv_convolve_noalloc", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_int64, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
     ("nv_save_vectors", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p]),
     ("nv_load_vectors", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64, _dp, _dp, _dp]),
     ("nv_save_archive", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]),
@@ -111,6 +114,13 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -m nadvibs_amd.build` (no CPU fallback exists)")
+    # PyTorch-ROCm bundles its own HIP runtime under the same soname: loaded first, it serves
+    # both (device pointers and streams then cross the binding); loaded second, it would meet
+    # the system runtime already in the process and find no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)
