@@ -321,6 +321,20 @@ nv_ctx *nv_batch_context(nv_batch *b, int64_t p);
 int32_t nv_convolve(nv_ctx *ctx, int64_t n, const double *x, const double *y,
                     int64_t ng, const double *grid, double sigma, double *env);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): nv_convolve on
+ * device arrays x_dev[n], y_dev[n], grid_dev[ng] into env_dev[ng], enqueued
+ * on `stream` (a hipStream_t, NULL = the context's stream), with its
+ * per-workgroup maxima in a static device array of the library instead of an
+ * allocation.  ng <= NV_CONVOLVE_MAX_GRID (else NV_ERR_ARG).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU race.  No host
+ * synchronisation. */
+#define NV_CONVOLVE_MAX_GRID 1048576
+int32_t nv_convolve_noalloc(nv_ctx *ctx, int64_t n, const double *x_dev,
+                            const double *y_dev, int64_t ng,
+                            const double *grid_dev, double sigma,
+                            double *env_dev, void *stream);
+
 /* ---- restart files (make_restart / load_restartinfo,
  *      Source/top.f90:1007-1049, Source/secondary.f90:872-892) -------------- */
 /* Writes q1, q2, q3 of the last iteration as nstates records each of dimen

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,35 @@ struct nv_ctx {
 };
 
 static std::string g_create_error;
+
+// ---- static scratch of the no-allocation entry point ------------------------
+// nv_convolve_noalloc keeps its per-workgroup maxima (one double per kBlock grid
+// points) here instead of in an allocation: 32 KiB per GPU, held until the
+// process ends.  In host code `scratch` only names the symbol; its address on
+// each device comes from hipGetSymbolAddress, taken once per device in
+// nv_create with that device current.
+constexpr int64_t kScratchDoubles = NV_CONVOLVE_MAX_GRID / nv::kBlock;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices] = {nullptr};
+static std::mutex g_scratch_mutex;
+
+// The address of `scratch` on the current device `device` (once per device).
+static hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipSuccess;  // the entry point then refuses
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[device] = static_cast<double *>(p);
+    return e;
+}
+
+static double *scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
 
 namespace {
 
@@ -158,6 +188,7 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
         c->jit_on = !(k && (std::strcmp(k, "generic") == 0 || std::strcmp(k, "steps") == 0));
     }
     hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = scratch_setup(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     const nv::Operator &op = c->op;
     NvOpDev &d = c->dev;
@@ -1474,6 +1505,23 @@ int32_t nv_convolve(nv_ctx *c, int64_t n, const double *x, const double *y, int6
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(d);
     if (e != hipSuccess) return hip_fail(c, e, "convolve");
+    return NV_OK;
+}
+
+int32_t nv_convolve_noalloc(nv_ctx *c, int64_t n, const double *x, const double *y, int64_t ng, const double *grid,
+                            double sigma, double *env, void *stream) {
+    if (!c) return NV_ERR_ARG;
+    if (n < 1 || ng < 1 || !x || !y || !grid || !env || !(sigma > 0))
+        return fail(c, NV_ERR_ARG, "bad convolve arguments");
+    if (ng > NV_CONVOLVE_MAX_GRID)
+        return fail(c, NV_ERR_ARG, "nv_convolve_noalloc takes at most NV_CONVOLVE_MAX_GRID (" +
+                                       std::to_string(NV_CONVOLVE_MAX_GRID) + ") grid points, got " +
+                                       std::to_string(ng));
+    double *part = scratch_of(c->device);
+    if (!part) return fail(c, NV_ERR_STATE, "the static scratch has no address on this device");
+    NV_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    NV_HIP(c, nv::launch_convolve(x, y, n, grid, ng, sigma, env, part, s));
     return NV_OK;
 }
 
