@@ -38,3 +38,16 @@ def test_compare_matches_each_converged_root_to_its_nearest():
     assert n == 3
     assert abs(dE - 1e-9 * bench.AU2WAVE) < 1e-9
     assert abs(dI - 1e-6) < 1e-12
+
+
+def test_compare_windows_separates_window_all_and_both_converged():
+    au = bench.AU2WAVE
+    ev64 = np.array([0.0, 1000.0, 5000.0, 7000.0, 2000.0]) / au
+    bj64 = np.array([1e-12, 1e-12, 1e-12, 1e-12, 1e-3])
+    # mixed run: the 1000 cm^-1 root off by 1e-4 cm^-1, the 7000 cm^-1 root still converging (off by 0.5)
+    ev = np.array([0.0, 1000.0001, 5000.0, 7000.5, 2000.0]) / au
+    bj = np.array([1e-12, 1e-12, 1e-12, 1e-4, 1e-12])
+    w = bench.compare_windows(ev64, bj64, ev, bj)
+    assert w["0_6000"][1] == 3 and abs(w["0_6000"][0] - 1e-4) < 1e-7
+    assert w["all"][1] == 4 and abs(w["all"][0] - 0.5) < 1e-7
+    assert w["both"][1] == 3 and abs(w["both"][0] - 1e-4) < 1e-7
