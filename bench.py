@@ -209,6 +209,7 @@ def adjoint_cfg4(sync):
             target = spectrum_of_parameters(p, w0, c0, pt0 * scale, niter, grid, 15.0, "lowest", reorthog=1)
         for run, kw in runs:
             out, grads, losses = {}, {}, {}
+            refused = None
             for prec in ("f64", "f32"):
                 w, c, pt = (t.clone().requires_grad_(True) for t in (w0, c0, pt0))
                 sync()
@@ -216,13 +217,22 @@ def adjoint_cfg4(sync):
                 env = spectrum_of_parameters(p, w, c, pt, niter, grid, 15.0, "lowest", prec=prec, **kw)
                 loss = torch.linalg.vector_norm(env - target)
                 t1 = time.perf_counter()
-                loss.backward()
+                import warnings
+                with warnings.catch_warnings(record=True) as caught:
+                    warnings.simplefilter("always")
+                    loss.backward()
                 t2 = time.perf_counter()
+                flags = [str(x.message) for x in caught if "orthogonality" in str(x.message)]
+                if flags:  # the library flags a basis that lost orthogonality
+                    refused = flags[0]
                 grads[prec] = torch.cat([w.grad.reshape(-1), c.grad.reshape(-1), pt.grad.reshape(-1)]).numpy()
                 losses[prec] = float(loss)
                 out[prec] = {"forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3), "loss": float(loss)}
                 print(f"# adjoint cfg4 {label} {run} {prec}: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s, "
                       f"loss {float(loss):.6g}", file=sys.stderr, flush=True)
+            if refused is not None:
+                out["orthogonality_warning"] = refused
+                print(f"# adjoint cfg4 {label} {run}: {refused}", file=sys.stderr, flush=True)
             d = grads["f32"] - grads["f64"]
             out["parameters"] = int(grads["f64"].size)
             out["nonzero_gradients"] = int(np.count_nonzero(grads["f64"]))

@@ -31,6 +31,11 @@ extern "C" {
 #define NV_ERR_STATE 3   /* call out of sequence (e.g. iteration before init) */
 #define NV_ERR_IO 4      /* file could not be read or written */
 #define NV_ERR_NOMEM 5   /* device or host allocation failed */
+#define NV_WARN_NONORTHOGONAL 6  /* nv_krylov_gradient only: the gradient is written,
+                                    but the Krylov basis has lost orthogonality
+                                    (an overlap |q_j . q_k| above 1e-4, named by
+                                    nv_last_error), so the projected gradient is
+                                    not the derivative of the computed loss */
 
 /* ---- operator conventions (reference quirks) --------------------------- */
 /* <m|q^3|m+1> taken as sqrt((m+1)^3/8), one third of the exact value
@@ -156,7 +161,9 @@ typedef struct nv_lanczos_opts {
     int64_t checkpoint;       /* K > 0 with neither savevecs nor reorthog: keep
                                  (q_{cK}, r_{cK}) every K iterations so that
                                  nv_lanczos_adjoint can recompute the basis
-                                 segment by segment; 0 = none */
+                                 segment by segment; 0 = none.  Refused with
+                                 savevecs or reorthog > 0: those keep every
+                                 vector (re-orthogonalisation reads them) */
     double pro_tau;           /* > 0: re-orthogonalisation trigger on |omega|
                                  instead of the default (sqrt(eps), or
                                  NV_PRO_MIXED_TAU in the mixed modes) */
@@ -278,7 +285,11 @@ int32_t nv_spectrum_adjoint(nv_ctx *ctx, int64_t n, const double *alpha,
  * segment from the checkpoints, each pair of segments (A, B >= A) once with A
  * resident (nseg (nseg + 1) / 2 segment recomputations of K iterations, each
  * checked bit for bit against the stored alpha/beta; 2 (K + 2) N doubles of
- * device scratch).  Output layouts as nv_lanczos_adjoint.  Synchronises. */
+ * device scratch).  Output layouts as nv_lanczos_adjoint.  The basis is
+ * checked on the way: the first, second, middle and last vectors (kept basis)
+ * or the first and last of every segment (checkpoints) against every later
+ * vector; an overlap above 1e-4 returns NV_WARN_NONORTHOGONAL with the
+ * gradient written.  Synchronises. */
 int32_t nv_krylov_gradient(nv_ctx *ctx, int64_t n, const double *tbar,
                            double *g_omega, double *g_concoef,
                            double *g_potterms);

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnadvibs.so")
 
 NV_OK = 0
+NV_WARN_NONORTHOGONAL = 6
 NV_QUIRK_Q3ELEM = 1
 NV_QUIRK_LOWERDIAG = 2
 NV_QUIRKS_REF = 3

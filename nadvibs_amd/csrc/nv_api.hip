@@ -552,6 +552,9 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     const bool keep = o->reorthog > 0 || o->savevecs != 0;
     // buffers of an earlier run with the same shape are reused
     if (o->checkpoint < 0) return fail(c, NV_ERR_ARG, "checkpoint must be >= 0");
+    if (o->checkpoint > 0 && keep)
+        return fail(c, NV_ERR_ARG, "checkpoints apply to runs without re-orthogonalisation or savevecs: those "
+                                   "keep every Lanczos vector (re-orthogonalisation reads them)");
     const int64_t K = keep ? 0 : o->checkpoint;
     const bool same = c->lz_ready && c->opts.niter == o->niter && c->opts.prec == o->prec &&
                       (keep ? 0 : c->opts.checkpoint) == K &&
@@ -1452,9 +1455,44 @@ int32_t nv_krylov_gradient(nv_ctx *c, int64_t n, const double *tbar, double *g_o
                 e = nv::launch_grad_contract(c->gdev, W + j * N, obuf + ((j0 + j) % ons) * N, gpart, gacc, s);
         }
     };
+    // Orthogonality of the basis the projection assumes: probe vectors against every later
+    // vector of a buffer, the largest |overlap| kept.
+    double *ppart = nullptr;
+    const int pch = nv::probe_chunks(N);
+    if (e == hipSuccess) e = alloc((size_t)std::max<int64_t>(n, 1) * pch * nv::kMaxProbes * 8, (void **)&ppart);
+    double worst = 0.0;
+    int64_t worst_j = 0, worst_k = 0;
+    auto probe = [&](const double *pbuf, int64_t pns, const std::vector<int64_t> &pj, const double *vbuf,
+                     int64_t vns, int64_t v0, int64_t v1) {
+        if (e != hipSuccess || v1 < v0) return;
+        nv::ProbeSet ps{};
+        ps.np = (int)pj.size();
+        for (int q = 0; q < ps.np; ++q) ps.p[q] = pbuf + (pj[q] % pns) * N;
+        const int64_t nj = v1 - v0 + 1;
+        e = nv::launch_probe_dots(ps, vbuf, vns, N, v0, nj, ppart, s);
+        std::vector<double> h((size_t)nj * pch * nv::kMaxProbes);
+        if (e == hipSuccess) e = hipMemcpyAsync(h.data(), ppart, h.size() * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return;
+        for (int64_t k = 0; k < nj; ++k)
+            for (int q = 0; q < ps.np; ++q) {
+                if (v0 + k <= pj[q]) continue;  // later vectors only (and not the probe itself)
+                double d = 0.0;
+                for (int ch = 0; ch < pch; ++ch) d += h[((size_t)k * pch + ch) * nv::kMaxProbes + q];
+                if (std::fabs(d) > worst) {
+                    worst = std::fabs(d);
+                    worst_j = pj[q];
+                    worst_k = v0 + k;
+                }
+            }
+    };
     if (archive) {
         // q_j in slot j of the archive
         contract(c->st.V, c->st.nslots, 1, n, c->st.V, c->st.nslots, 1, n);
+        std::vector<int64_t> pj;
+        for (int64_t j : {int64_t(1), int64_t(2), (n + 2) / 3, (2 * n + 2) / 3, n - 1})
+            if (j >= 1 && j < n && std::find(pj.begin(), pj.end(), j) == pj.end()) pj.push_back(j);
+        probe(c->st.V, c->st.nslots, pj, c->st.V, c->st.nslots, 2, n);
     } else {
         // Segment pairs (A, B), B >= A, from the checkpoints: segment A stays resident while
         // every later segment is recomputed beside it; an off-diagonal pair contributes both
@@ -1466,14 +1504,17 @@ int32_t nv_krylov_gradient(nv_ctx *c, int64_t n, const double *tbar, double *g_o
             const int64_t a0 = A * K + 1, a1 = std::min((A + 1) * K, n);
             for (int64_t B = A; B < nseg && e == hipSuccess; ++B) {
                 const int64_t b0 = B * K + 1, b1 = std::min((B + 1) * K, n);
+                const std::vector<int64_t> pj = a1 > a0 ? std::vector<int64_t>{a0, a1} : std::vector<int64_t>{a0};
                 if (B == A) {
                     contract(segA, K + 2, a0, a1, segA, K + 2, a0, a1);
+                    probe(segA, K + 2, pj, segA, K + 2, a0 + 1, a1);
                     continue;
                 }
                 rc = recompute_segment(c, B, K, n, segB, ta, tbb);
                 if (rc != NV_OK) return done(rc);
                 contract(segA, K + 2, a0, a1, segB, K + 2, b0, b1);
                 contract(segB, K + 2, b0, b1, segA, K + 2, a0, a1);
+                probe(segA, K + 2, pj, segB, K + 2, b0, b1);
             }
         }
     }
@@ -1482,6 +1523,14 @@ int32_t nv_krylov_gradient(nv_ctx *c, int64_t n, const double *tbar, double *g_o
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return done(hip_fail(c, e, "krylov gradient"));
     map_gradients(c, gh, g_omega, g_concoef, g_potterms);
+    if (worst > 1e-4) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "the Krylov basis has lost orthogonality: |q_%lld . q_%lld| = %.3e > 1e-4; the projected "
+                      "gradient is not the derivative of the computed loss (re-orthogonalise, or shorten the run)",
+                      (long long)worst_j, (long long)worst_k, worst);
+        return done(fail(c, NV_WARN_NONORTHOGONAL, msg));
+    }
     return done(NV_OK);
 }
 

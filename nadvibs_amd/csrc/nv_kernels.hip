@@ -12,6 +12,8 @@
 //  * tri_*         Sturm bisection and inverse iteration for the tridiagonal
 //                  eigenproblem, one root per thread.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <hip/hip_bf16.h>
 
 #include <type_traits>
@@ -1569,6 +1571,42 @@ __global__ __launch_bounds__(kBlock) void convolve_norm_kernel(double *env, int6
     __syncthreads();
     for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < ng; g += (int64_t)gridDim.x * kBlock)
         env[g] /= mx;
+}
+
+// Overlaps of up to kMaxProbes probe vectors with basis vectors j0..j0+nj-1
+// (q_j in slot j % nslots of V): grid (chunks, nj), partial dots per chunk in
+// part[(k * chunks + chunk) * kMaxProbes + p], summed on the host.
+__global__ __launch_bounds__(kBlock) void probe_dots_kernel(ProbeSet ps, const double *__restrict__ V, int64_t nslots,
+                                                            int64_t N, int64_t j0, int64_t chunk,
+                                                            double *__restrict__ part) {
+    const int64_t k = blockIdx.y;
+    const double *q = V + ((j0 + k) % nslots) * N;
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < N ? lo + chunk : N;
+    double acc[kMaxProbes];
+#pragma unroll
+    for (int p = 0; p < kMaxProbes; ++p) acc[p] = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const double v = q[i];
+#pragma unroll
+        for (int p = 0; p < kMaxProbes; ++p)
+            if (p < ps.np) acc[p] += v * ps.p[p][i];
+    }
+#pragma unroll
+    for (int p = 0; p < kMaxProbes; ++p) {
+        const double t = block_sum(acc[p]);
+        if (threadIdx.x == 0) part[(k * gridDim.x + blockIdx.x) * kMaxProbes + p] = t;
+    }
+}
+
+int probe_chunks(int64_t N) { return (int)std::min<int64_t>(64, (N + 65535) / 65536); }
+
+hipError_t launch_probe_dots(const ProbeSet &ps, const double *V, int64_t nslots, int64_t N, int64_t j0, int64_t nj,
+                             double *part, hipStream_t s) {
+    const int nc = probe_chunks(N);
+    const int64_t chunk = (N + nc - 1) / nc;
+    hipLaunchKernelGGL(probe_dots_kernel, dim3(nc, (unsigned)nj), dim3(kBlock), 0, s, ps, V, nslots, N, j0, chunk,
+                       part);
+    return hipGetLastError();
 }
 
 hipError_t launch_convolve(const double *x, const double *y, int64_t n, const double *grid, int64_t ng,

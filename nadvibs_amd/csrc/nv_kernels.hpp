@@ -91,6 +91,17 @@ hipError_t launch_norm_to(const double *v, int64_t n, double *out, double *parti
 
 // Normalised Gaussian envelope of sticks (x, y) on a grid; part holds
 // ceil(ng/256) doubles.
+// Dots of up to kMaxProbes probe vectors (length N) with the basis vectors
+// q_j0..q_{j0+nj-1} (slot j % nslots of V); part holds nj * probe_chunks(N) *
+// kMaxProbes partial sums, summed on the host per (j, probe).
+constexpr int kMaxProbes = 8;
+struct ProbeSet {
+    const double *p[kMaxProbes];
+    int np;
+};
+int probe_chunks(int64_t N);
+hipError_t launch_probe_dots(const ProbeSet &ps, const double *V, int64_t nslots, int64_t N, int64_t j0, int64_t nj,
+                             double *part, hipStream_t s);
 hipError_t launch_convolve(const double *x, const double *y, int64_t n, const double *grid, int64_t ng,
                            double sigma, double *env, double *part, hipStream_t s);
 

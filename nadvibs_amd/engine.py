@@ -242,15 +242,27 @@ class Engine:
                                                  dptr(ab), dptr(bb), dptr(tb)))
         return (ab, bb, tb) if full else (ab, bb)
 
-    def krylov_gradient(self, tbar):
+    def krylov_gradient(self, tbar, strict=False):
         """dL/d(aomega, concoef, potterms) of a spectral loss with dL/dT = tbar,
-        projected on the kept Krylov basis (nv_krylov_gradient)."""
+        projected on the kept Krylov basis (nv_krylov_gradient).  A basis that
+        has lost orthogonality (status NV_WARN_NONORTHOGONAL) gives a
+        RuntimeWarning naming the overlap, or NvError when strict; the message
+        is kept in self.orthogonality_warning (None when the basis passed).
+        Such a projection can still match the derivative of the computed loss
+        (short runs whose spectrum has converged), but nothing guarantees it."""
         tbar = np.ascontiguousarray(tbar, dtype=np.float64)
         n = tbar.shape[0]
         p = self.prob
         go, gc = np.zeros(p.nmodes), np.zeros(p.nstblks)
         gp = np.zeros(np.shape(p.potterms))
-        self._check(self.lib.nv_krylov_gradient(self.h, n, dptr(tbar), dptr(go), dptr(gc), dptr(gp)))
+        rc = self.lib.nv_krylov_gradient(self.h, n, dptr(tbar), dptr(go), dptr(gc), dptr(gp))
+        self.orthogonality_warning = None
+        if rc == _lib.NV_WARN_NONORTHOGONAL and not strict:
+            import warnings
+            self.orthogonality_warning = self.lib.nv_last_error(self.h).decode()
+            warnings.warn(self.orthogonality_warning, RuntimeWarning)
+        else:
+            self._check(rc)
         return dict(omega=go, concoef=gc, potterms=gp)
 
     def hv_kernel(self, prec="f64"):

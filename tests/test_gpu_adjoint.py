@@ -238,3 +238,34 @@ def test_spectrum_of_parameters_matches_central_differences():
         gots.append(float(vals[name].grad[idx]))
     fds, gots = np.array(fds), np.array(gots)
     assert np.max(np.abs(gots - fds)) <= 1e-6 * np.max(np.abs(fds)), (gots, fds)
+
+
+def test_krylov_gradient_reports_a_basis_that_lost_orthogonality():
+    """nv_krylov_gradient checks the basis it projects on: a run without
+    re-orthogonalisation long enough to lose orthogonality returns
+    NV_WARN_NONORTHOGONAL (a warning in the binding, NvError when strict),
+    a re-orthogonalised run passes; checkpoints with re-orthogonalisation are
+    refused (that run keeps every vector)."""
+    from nadvibs_amd._lib import NvError
+    p = _prob(ns=3, nf=(4, 4, 3, 3), seed=36, quirks=0)
+    n = 60
+    grid = np.linspace(-100.0, 4000.0, 400)
+    for reorthog in (0, 1):
+        with Engine(p) as eng:
+            eng.lanczos_init(n, reorthog=reorthog, prec="f64", savevecs=True)
+            eng.run(1, n)
+            a, b, _ = eng.coefficients()
+            env = eng.spectrum(a, b, grid, 30.0, "fixed", 0.0)
+            _, _, tb = eng.spectrum_adjoint(a, b, grid, 30.0, 2 * env, "fixed", 0.0, full=True)
+            if reorthog == 0:
+                with pytest.warns(RuntimeWarning, match="lost orthogonality"):
+                    eng.krylov_gradient(tb)
+                with pytest.raises(NvError, match="lost orthogonality") as ex:
+                    eng.krylov_gradient(tb, strict=True)
+                assert ex.value.code == 6
+            else:
+                eng.krylov_gradient(tb, strict=True)
+                assert eng.orthogonality_warning is None
+        with Engine(p) as eng:
+            with pytest.raises(NvError, match="checkpoints apply"):
+                eng.lanczos_init(n, reorthog=1, prec="f64", checkpoint=10)
