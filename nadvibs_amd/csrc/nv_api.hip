@@ -295,7 +295,8 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
         free_list(c->allocs);
         hipStreamDestroy(c->stream);
         delete c;
-        return fail(nullptr, NV_ERR_ARG, "factor table exceeds the LDS budget of the H*v kernel");
+        return fail(nullptr, NV_ERR_ARG,
+                    "factor table and mode digits exceed the 64 KiB LDS budget of the H*v kernel");
     }
     *out = c;
     return NV_OK;

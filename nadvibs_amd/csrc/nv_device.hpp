@@ -14,7 +14,7 @@
 #include <stdint.h>
 
 #define NV_MAXF 4          // factors per step (ordr <= 4)
-#define NV_MAXM 64         // modes
+#define NV_MAXM 100        // modes (basis(100) of the reference's namelist)
 #define NV_MAXS 10         // states (basis.in weights has 10 slots)
 
 // The FP32/BF16 copy of a Lanczos vector that H*v reads is state-interleaved,

@@ -221,7 +221,7 @@ static void build_two_pass(Operator &op, CodeOf &code_of) {
 
 bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
     if (pr.nstates < 1 || pr.nstates > NV_MAXS) { err = "nstates must be in 1..10"; return false; }
-    if (pr.nmodes < 1 || pr.nmodes > NV_MAXM) { err = "nmodes must be in 1..64"; return false; }
+    if (pr.nmodes < 1 || pr.nmodes > NV_MAXM) { err = "nmodes must be in 1.." + std::to_string(NV_MAXM); return false; }
     if (pr.ordr < 1 || pr.ordr > NV_MAXF) { err = "ordr must be in 1..4"; return false; }
     if (!pr.nfunc || !pr.aomega || !pr.concoef || !pr.potterms) { err = "null problem array"; return false; }
     op.ns = pr.nstates; op.M = pr.nmodes; op.ordr = pr.ordr; op.quirks = pr.quirks;

@@ -56,8 +56,10 @@ def dense_hamiltonian(prob, dtype=float):
     D = int(np.prod(nf))
     H = np.zeros((ns * D, ns * D), dtype=dtype)
     # harmonic part
-    grids = np.meshgrid(*[np.arange(v) for v in nf], indexing="ij")
-    E = sum((g.reshape(-1) + 0.5) * w for g, w in zip(grids, prob.aomega))
+    # digits of every row (last mode fastest), without an nmodes-dimensional array
+    r = np.arange(D)
+    shft = [int(np.prod(nf[k + 1:])) for k in range(nm)]
+    E = sum(((r // shft[k]) % nf[k] + 0.5) * prob.aomega[k] for k in range(nm))
     for s in range(ns):
         H[s * D:(s + 1) * D, s * D:(s + 1) * D] += np.diag(E)
     blocks = [(i, j) for i in range(ns) for j in range(i + 1)]

@@ -30,6 +30,7 @@
 #include "../include/nadvibs.h"
 
 #define MAXF 4
+#define MAXM 100  /* modes: basis(100) of the reference's namelist (Source/secondary.f90:830-833) */
 
 typedef struct {
     int cnt;
@@ -124,7 +125,7 @@ void or_destroy(OOp *op) {
 }
 
 OOp *or_create(const nv_problem *pr) {
-    if (pr->ordr < 1 || pr->ordr > MAXF || pr->nstates < 1 || pr->nmodes < 1) return NULL;
+    if (pr->ordr < 1 || pr->ordr > MAXF || pr->nstates < 1 || pr->nmodes < 1 || pr->nmodes > MAXM) return NULL;
     OOp *op = (OOp *)calloc(1, sizeof(OOp));
     op->ns = (int)pr->nstates; op->M = (int)pr->nmodes; op->ordr = (int)pr->ordr; op->quirks = pr->quirks;
     op->nstblks = op->ns * (op->ns + 1) / 2;
@@ -346,7 +347,7 @@ void or_hv(const OOp *op, const double *x, double *y) {
     const int lower = (op->quirks & NV_QUIRK_LOWERDIAG) != 0;
     memset(y, 0, (size_t)op->N * sizeof(double));
     /* diagonal rows: harmonic energy by the odometer of :717-731 */
-    long barray[64] = {0};
+    long barray[MAXM] = {0};
     for (int j = 0; j < M; ++j) barray[j] = 1;
     barray[M - 1] -= 1;
     double dkin = 0.0;
@@ -440,7 +441,7 @@ static void o_row(const OOp *op, const double *x, long r, double *yr) {
     const int ns = op->ns, M = op->M;
     const long D = op->D;
     const int lower = (op->quirks & NV_QUIRK_LOWERDIAG) != 0;
-    long nd[64];
+    long nd[MAXM];
     double dkin = 0.0;
     for (int j = 0; j < M; ++j) {
         nd[j] = (r / op->shft[j]) % op->nfunc[j];

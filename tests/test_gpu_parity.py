@@ -465,3 +465,27 @@ def test_two_pass_hv_matches_oracle(case, variant, monkeypatch):
             a, b, _ = eng.coefficients()
             assert np.max(np.abs(a - r["alpha"]) / np.abs(r["alpha"])) <= tol_ab, prec
             assert np.max(np.abs(b - r["beta"]) / np.abs(r["beta"])) <= tol_ab, prec
+
+
+@pytest.mark.parametrize("kernel", ["default", "generic", "steps"])
+def test_hundred_mode_operator_matches_oracle(kernel, monkeypatch):
+    """100 modes (the reference's basis(100)), 96 of them with one function:
+    H*v in both precisions and the first coefficients against the oracle, in
+    every kernel form."""
+    monkeypatch.delenv("NV_HV_KERNEL", raising=False)
+    if kernel != "default":
+        monkeypatch.setenv("NV_HV_KERNEL", kernel)
+    p = make_problem(2, [3, 4, 2, 3] + [1] * 96, 3, (400, 3000), 250.0, 60.0, 40.0, cubic=10.0, seed=9)
+    orc = Oracle(p)
+    x = np.random.default_rng(2).standard_normal(p.n)
+    ref = orc.hv(x)
+    r = orc.lanczos(12)
+    with Engine(p) as eng:
+        for prec, tol_hv, tol_ab in (("f64", 1e-13, 1e-11), ("f32", 2e-6, 5e-6)):
+            err = np.max(np.abs(eng.hv(x, prec) - ref)) / np.max(np.abs(ref))
+            assert err <= tol_hv, (prec, err, eng.hv_kernel(prec))
+            eng.lanczos_init(niter=12, reorthog=0, prec=prec)
+            eng.run(1, 12)
+            a, b, _ = eng.coefficients()
+            assert np.max(np.abs(a - r["alpha"]) / np.abs(r["alpha"])) <= tol_ab, prec
+            assert np.max(np.abs(b - r["beta"]) / np.abs(r["beta"])) <= tol_ab, prec

@@ -285,3 +285,34 @@ def test_parallel_hv_and_recurrence_match_serial():
     st, a2, b2 = o.lanczos_par(40, 2, st, a2, b2, i0=21)
     assert np.max(np.abs(a2 - r["alpha"]) / np.abs(r["alpha"])) < 1e-11
     assert np.max(np.abs(b2 - r["beta"]) / np.abs(r["beta"])) < 1e-11
+
+
+def test_hundred_modes_pinned_and_accepted():
+    """The reference's namelist holds basis(100) (Source/secondary.f90:830-833):
+    a 100-mode operator (97 modes with one function) against the dense brute
+    force, and the library's operator build accepts it (without a GPU its
+    device setup fails after the validation)."""
+    import ctypes
+
+    from nadvibs_amd import _lib
+    p = make_problem(2, [3, 2, 3] + [1] * 97, 2, (400, 3000), 250.0, 60.0, 40.0, seed=9)
+    H = dense_hamiltonian(p)
+    x = np.random.default_rng(1).standard_normal(p.n)
+    ref = H @ x
+    assert np.max(np.abs(Oracle(p).hv(x) - ref)) <= 1e-13 * np.max(np.abs(ref))
+    lib = _lib.load()
+    keep = [np.ascontiguousarray(p.nfunc, dtype=np.int64), np.ascontiguousarray(p.aomega),
+            np.ascontiguousarray(p.concoef), np.ascontiguousarray(p.potterms)]
+    desc = _lib.NvProblem(nstates=p.nstates, nmodes=p.nmodes, ordr=p.ordr, nfunc=_lib.iptr(keep[0]),
+                          aomega=_lib.dptr(keep[1]), concoef=_lib.dptr(keep[2]), potterms=_lib.dptr(keep[3]),
+                          ztoler=1e-20, quirks=3, nirreps=1)
+    h = ctypes.c_void_p()
+    rc = lib.nv_create(ctypes.byref(desc), 0, ctypes.byref(h))
+    if rc == 0:
+        lib.nv_destroy(h)
+    else:
+        msg = lib.nv_last_error(None).decode()
+        assert rc == 2 and "nmodes" not in msg, (rc, msg)
+    desc.nmodes = 101
+    assert lib.nv_create(ctypes.byref(desc), 0, ctypes.byref(h)) == 1
+    assert "nmodes" in lib.nv_last_error(None).decode()
