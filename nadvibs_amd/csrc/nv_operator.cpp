@@ -245,6 +245,9 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
     // ---- term numbering (Source/top.f90:150-201) ----
     const int64_t ntmax = noterms(op.M, op.ordr);
     std::vector<std::vector<Term>> bycnt(op.ordr + 1);
+    // (modes, powers) -> position in bycnt[cnt], in first-seen order (the reference's
+    // grouping; a map keeps it O(T log T) for many modes)
+    std::vector<std::map<uint64_t, size_t>> seen(op.ordr + 1);
     for (int64_t b = 0; b < op.nstblks; ++b) {
         for (int64_t p = 1; p <= op.ordr; ++p) {
             auto mons = monomials(op.M, p);
@@ -259,13 +262,14 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
                     if (f < 0) { um[cnt] = v; up[cnt] = 1; ++cnt; } else up[f]++;
                 }
                 auto &list = bycnt[cnt];
+                uint64_t key = 0;
+                for (int j = 0; j < cnt; ++j) key = (key << 11) | ((uint64_t)um[j] << 3) | (uint64_t)up[j];
+                auto found = seen[cnt].find(key);
                 Term *t = nullptr;
-                for (auto &e : list) {
-                    bool same = true;
-                    for (int j = 0; j < cnt; ++j) same = same && e.mode[j] == um[j] && e.pow[j] == up[j];
-                    if (same) { t = &e; break; }
-                }
-                if (!t) {
+                if (found != seen[cnt].end()) {
+                    t = &list[found->second];
+                } else {
+                    seen[cnt][key] = list.size();
                     list.emplace_back();
                     t = &list.back();
                     t->cnt = cnt;

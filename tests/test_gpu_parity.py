@@ -475,7 +475,10 @@ def test_hundred_mode_operator_matches_oracle(kernel, monkeypatch):
     monkeypatch.delenv("NV_HV_KERNEL", raising=False)
     if kernel != "default":
         monkeypatch.setenv("NV_HV_KERNEL", kernel)
-    p = make_problem(2, [3, 4, 2, 3] + [1] * 96, 3, (400, 3000), 250.0, 60.0, 40.0, cubic=10.0, seed=9)
+    # ordr 2: all 5050 quadratic monomials of 100 modes carry couplings (the checker groups the
+    # terms by a linear search, as the reference does, so cubic couplings on every mode would take
+    # it minutes)
+    p = make_problem(2, [3, 4, 2, 3] + [1] * 96, 2, (400, 3000), 250.0, 60.0, 40.0, seed=9)
     orc = Oracle(p)
     x = np.random.default_rng(2).standard_normal(p.n)
     ref = orc.hv(x)
