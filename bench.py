@@ -128,6 +128,11 @@ def batched_cfg5(sync, cpu_seconds):
         a0, _ = bt.coefficients(0)
         t_batch = time.perf_counter() - t0
     print(f"# batched cfg5: {t_batch * 1e3:.1f} ms ({rep})", file=sys.stderr, flush=True)
+    # the chosen form's batched H*v time (x read in FP32, y written in FP32 per element)
+    import re
+    times = {m.group(1): float(m.group(2)) for m in re.finditer(r" (\d+/\d/\w+) ([\d.]+) us;", rep)}
+    chosen = re.search(r"chosen (\S+)", rep).group(1)
+    hv_us = times.get(chosen, float("nan"))
     engs = []
     for p in probs:
         e = Engine(p)
@@ -174,6 +179,7 @@ def batched_cfg5(sync, cpu_seconds):
             "speedup_vs_sequential": round(t_seq / t_batch, 2), "speedup_vs_streams": round(t_streams / t_batch, 2),
             "problem_iter_per_s": round(len(probs) * niter / t_batch, 1),
             "alpha1_agree": bool(abs(a0[0] - a1[0]) <= 1e-12 * abs(a1[0])), "tune": rep,
+            "batched_hv_us": round(hv_us, 1), "batched_hv_gbps": round(total * 8 / hv_us / 1e3, 1),
             "cpu_baseline": {"value": round(len(probs) * k / dt, 2), "unit": "problem-iterations/s", "cores": 1,
                              "kind": "port", "sample": f"all 64 problems, {k} iterations each (FP64 oracle), "
                                                        f"{dt:.1f} s"}}

@@ -314,9 +314,10 @@ const char *nv_batch_last_error(const nv_batch *b);
 /* Allocates the vectors of niter iterations in precision prec (NV_PREC_F64
  * or NV_PREC_F32) and loads every start vector. */
 int32_t nv_batch_init(nv_batch *b, int64_t niter, int32_t prec);
-/* Times the batched H*v at 64, 128 and 256 rows per workgroup and keeps the
- * fastest; call between nv_batch_init and the run.  report (may be NULL)
- * receives the timings. */
+/* Times the batched H*v over its tuning space -- 64, 128 and 256 rows per
+ * workgroup, one or two rows per thread (FP32, from 128 rows), the shaped or
+ * the step-table kernels -- and keeps the fastest; call between
+ * nv_batch_init and the run.  report (may be NULL) receives the timings. */
 int32_t nv_batch_tune(nv_batch *b, char *report, int64_t len);
 /* Iterations iiter..niter for every problem.  Asynchronous. */
 int32_t nv_batch_run(nv_batch *b, int64_t iiter, int64_t niter);

@@ -45,8 +45,8 @@ class Batch:
         self.niter = niter
 
     def tune(self):
-        buf = ctypes.create_string_buffer(512)
-        self._check(self.lib.nv_batch_tune(self.h, buf, 512))
+        buf = ctypes.create_string_buffer(2048)
+        self._check(self.lib.nv_batch_tune(self.h, buf, 2048))
         return buf.value.decode()
 
     def run(self, iiter=1, niter=None):

@@ -2042,16 +2042,32 @@ int32_t nv_batch_tune(nv_batch *b, char *report, int64_t len) {
     if (!b) return NV_ERR_ARG;
     if (!b->ready || b->last != 0) return bfail(b, NV_ERR_STATE, "tune after nv_batch_init, before the run");
     NV_BHIP(b, hipSetDevice(b->device));
-    const int cand[3] = {64, 128, 256};
+    // the search space: rows per workgroup (tile size) x rows per thread (FP32) x kernel form
+    struct Cand {
+        int bs, rows, steps;
+    };
+    std::vector<Cand> cands;
+    const bool shaped = b->d.shaped[b->prec == NV_PREC_F64 ? 0 : 1] != 0;
+    for (int steps = 0; steps <= 1; ++steps) {
+        if (steps == 0 && !shaped) continue;
+        for (int bs : {64, 128, 256})
+            for (int rows = 1; rows <= 2; ++rows) {
+                if (rows == 2 && (b->prec == NV_PREC_F64 || bs < 128)) continue;
+                cands.push_back({bs, rows, steps});
+            }
+    }
     float best_ms = 0.0f;
-    int best = b->bs;
-    std::string rep = "batched H*v, rows per workgroup:";
+    Cand best = {b->bs, b->d.rows, b->d.steps};
+    std::string rep = "batched H*v (rows per workgroup / rows per thread / form):";
     hipEvent_t e0, e1;
     NV_BHIP(b, hipEventCreate(&e0));
     NV_BHIP(b, hipEventCreate(&e1));
     hipError_t e = hipSuccess;
-    for (int k = 0; k < 3 && e == hipSuccess; ++k) {
-        e = batch_tiles(b, cand[k]);
+    for (size_t k = 0; k < cands.size() && e == hipSuccess; ++k) {
+        const Cand &cd = cands[k];
+        e = batch_tiles(b, cd.bs);
+        b->d.rows = cd.rows;
+        b->d.steps = cd.steps;
         if (e == hipSuccess) e = nv::launch_set_iter(b->iter_d, 1, 0, b->stream);
         auto hv = [&]() {
             return nv::launch_batch_hv(b->d, b->groups.data(), (int)b->groups.size(), b->prec, b->lds_hv, b->stream);
@@ -2064,22 +2080,27 @@ int32_t nv_batch_tune(nv_batch *b, char *report, int64_t len) {
         float ms = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
         ms /= 10.0f;
-        char buf[64];
-        std::snprintf(buf, sizeof(buf), " %d: %.1f us;", cand[k], 1000.0 * ms);
+        char buf[96];
+        std::snprintf(buf, sizeof(buf), " %d/%d/%s %.1f us;", cd.bs, cd.rows, cd.steps ? "steps" : "shaped",
+                      1000.0 * ms);
         rep += buf;
         if (k == 0 || ms < best_ms) {
             best_ms = ms;
-            best = cand[k];
+            best = cd;
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    if (e == hipSuccess) e = batch_tiles(b, best);
+    if (e == hipSuccess) e = batch_tiles(b, best.bs);
+    b->d.rows = best.rows;
+    b->d.steps = best.steps;
     // alpha(1) of the timing launches is recomputed by the first iteration
     if (e == hipSuccess) e = hipMemsetAsync(b->d.alpha, 0, (size_t)b->ctx.size() * b->d.ld * sizeof(double), b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return bfail(b, NV_ERR_HIP, std::string("batch tune: ") + hipGetErrorString(e));
-    rep += " chosen " + std::to_string(best);
+    char buf[96];
+    std::snprintf(buf, sizeof(buf), " chosen %d/%d/%s", best.bs, best.rows, best.steps ? "steps" : "shaped");
+    rep += buf;
     if (report && len > 0) std::snprintf(report, (size_t)len, "%s", rep.c_str());
     return NV_OK;
 }

@@ -2108,13 +2108,16 @@ hipError_t launch_batch_normalize(const BatchDev &b, int prec, hipStream_t s) {
     return hipGetLastError();
 }
 
+// The tuned form of the batched H*v: shaped or step-table kernels, and in FP32
+// one or two rows per thread (two rows need workgroups of at least one wave).
 template <int NS>
 static void batch_hv_ns(const BatchDev &b, int prec, size_t lds, int t0, int nt, hipStream_t s) {
+    const bool two = prec != 0 && b.rows == 2 && b.bs >= 128;
     if constexpr (NS <= kShapedMaxStates) {
-        if (b.shaped[prec == 0 ? 0 : 1]) {
+        if (b.shaped[prec == 0 ? 0 : 1] && !b.steps) {
             if (prec == 0)
                 hipLaunchKernelGGL((batch_shaped_kernel<NS, double, double, 1>), dim3(nt), dim3(b.bs), lds, s, b, t0);
-            else if (b.bs >= 128)
+            else if (two)
                 hipLaunchKernelGGL((batch_shaped_kernel<NS, float, float, 2>), dim3(nt), dim3(b.bs / 2), lds, s, b,
                                    t0);
             else
@@ -2122,10 +2125,9 @@ static void batch_hv_ns(const BatchDev &b, int prec, size_t lds, int t0, int nt,
             return;
         }
     }
-    // FP32: two rows per thread, as for large single operators (workgroups of at least one wave)
     if (prec == 0)
         hipLaunchKernelGGL((batch_hv_kernel<NS, double, double>), dim3(nt), dim3(b.bs), lds, s, b, t0);
-    else if (b.bs >= 128)
+    else if (two)
         hipLaunchKernelGGL((batch_hv2_kernel<NS, float, float>), dim3(nt), dim3(b.bs / 2), lds, s, b, t0);
     else
         hipLaunchKernelGGL((batch_hv_kernel<NS, float, float>), dim3(nt), dim3(b.bs), lds, s, b, t0);

@@ -86,3 +86,26 @@ def test_batch_uneven_workload_sample():
         a1, b1 = _single(probs[i], n, "f32")
         np.testing.assert_allclose(got[i][0], a1, rtol=1e-8)
         np.testing.assert_allclose(got[i][1], b1, rtol=1e-8)
+
+
+def test_batch_tuner_searches_three_parameters():
+    """The FP32 tuning space: rows per workgroup x rows per thread x kernel
+    form (shaped or step table), every candidate timed; the chosen form gives
+    the oracle's first coefficients."""
+    probs = _probs()
+    n = 12
+    with Batch(probs) as bt:
+        bt.init(n, "f32")
+        rep = bt.tune()
+        print(rep, flush=True)
+        cands = [c for c in rep.split(":", 1)[1].split(";") if " us" in c]
+        assert len(cands) >= 8, rep
+        assert any("/2/" in c for c in cands) and any("steps" in c for c in cands), rep
+        assert "shaped" in rep and "chosen" in rep
+        bt.run(1, n)
+        res = [bt.coefficients(i) for i in range(len(probs))]
+    for p, (a, b) in zip(probs, res):
+        k = min(8, p.n // 4)
+        r = Oracle(p).lanczos(n)
+        np.testing.assert_allclose(a[:k], r["alpha"][:k], rtol=5e-6)
+        np.testing.assert_allclose(b[:k], r["beta"][:k], rtol=5e-6)
