@@ -83,24 +83,24 @@ def compare(ev64, it64, bj64, ev, it, bj, tol_conv=1e-9):
 def compare_windows(ev64, bj64, ev, bj, tol_conv=1e-9):
     """Eigenvalue error of the converged roots (bji < tol_conv hartree in the
     FP64 run), each matched to the nearest root of the other run: over the
-    0-6000 cm^-1 window above the lowest root, over every converged root, and
-    over the roots converged in both runs.  Without re-orthogonalisation a
-    root converged in one run can still be converging in the other (its Ritz
-    value then differs by up to its residual), which the last figure leaves
-    out."""
+    0-6000 cm^-1 window above the lowest root and over every converged root,
+    each once for every FP64-converged root and once for the roots converged
+    in both runs.  Without re-orthogonalisation a root converged in one run
+    can still be converging in the other (its Ritz value then differs by up
+    to its residual), which the second form leaves out."""
     rel = (ev64 - ev64[0]) * AU2WAVE
     out = {}
-    for name, sel in (("0_6000", (bj64 < tol_conv) & (rel <= 6000.0)), ("all", bj64 < tol_conv)):
-        idx = np.nonzero(sel)[0]
-        d = [abs(ev[int(np.argmin(np.abs(ev - ev64[k])))] - ev64[k]) * AU2WAVE for k in idx]
-        out[name] = (max(d) if d else float("nan"), int(idx.size))
-    d, n = [], 0
-    for k in np.nonzero(bj64 < tol_conv)[0]:
-        j = int(np.argmin(np.abs(ev - ev64[k])))
-        if bj[j] < tol_conv:
-            d.append(abs(ev[j] - ev64[k]) * AU2WAVE)
-            n += 1
-    out["both"] = (max(d) if d else float("nan"), n)
+    for name, win in (("0_6000", rel <= 6000.0), ("all", np.ones(ev64.size, bool))):
+        d, db, nb = [], [], 0
+        for k in np.nonzero((bj64 < tol_conv) & win)[0]:
+            j = int(np.argmin(np.abs(ev - ev64[k])))
+            e = abs(ev[j] - ev64[k]) * AU2WAVE
+            d.append(e)
+            if bj[j] < tol_conv:
+                db.append(e)
+                nb += 1
+        out[name] = (max(d) if d else float("nan"), len(d))
+        out[name + "_both" if name != "all" else "both"] = (max(db) if db else float("nan"), nb)
     return out
 
 
@@ -264,7 +264,7 @@ def load_traffic(path):
 def pmc_probe(args):
     """Kernels for the HBM-traffic counters (run under rocprofv3 --pmc):
     a device copy of known size (the calibration) and repeated H*v launches
-    of the mixed job's first iteration."""
+    of the first iteration, FP32 then FP64."""
     import torch
     from nadvibs_amd.engine import Engine
     from nadvibs_amd.synth import workload
@@ -274,16 +274,18 @@ def pmc_probe(args):
     for _ in range(args.steps):
         dst.copy_(src)
     torch.cuda.synchronize()
+    kernels = {}
     with Engine(prob, device=0) as eng:
-        kernel = eng.hv_kernel("f32")
-        eng.lanczos_init(niter=4, reorthog=0, prec="f32")
-        eng.update_lanczos(1)
-        for _ in range(args.steps):
-            eng.Hv(1)
-        eng.lanczos_step(1)
-        eng.coefficients(n=1)
+        for prec in ("f32", "f64"):
+            kernels[prec] = eng.hv_kernel(prec)
+            eng.lanczos_init(niter=4, reorthog=0, prec=prec)
+            eng.update_lanczos(1)
+            for _ in range(args.steps):
+                eng.Hv(1)
+            eng.lanczos_step(1)
+            eng.coefficients(n=1)
     print(json.dumps({"copy_bytes": 64 * 1024 * 1024, "hv_launches": args.steps, "n": prob.n,
-                      "hv_kernel": kernel}), flush=True)
+                      "hv_kernels": kernels}), flush=True)
 
 
 def lscpu_model():
@@ -531,19 +533,24 @@ def main():
         ev64, it64, bj64 = sol64
         ev32, it32, bj32 = sol32
         win = compare_windows(ev64, bj64, ev32, bj32)
-        dE, nconv = win["0_6000"]
+        dE, nboth = win["0_6000_both"]
         rec["fp64"] = {"value": round(v64, 3), "ms_per_step": round(dt64 / args.steps * 1e3, 3),
                        "hv_avg_launch_us": round(burst["f64"], 1),
                        "hv_gbps": round(N * 16 / burst["f64"] / 1e3, 2),
                        "in_job_hv_avg_launch_us": round(hv64_ms / max(hv64_n, 1) * 1e3, 1)}
         rec["speedup_vs_fp64"] = round(value / v64, 3)
         rec["max_abs_dE_cm1"] = float(f"{dE:.3e}")
-        rec["n_converged_fp64"] = nconv
+        rec["n_converged_both"] = nboth
+        rec["n_converged_fp64"] = win["0_6000"][1]
         rec["error_target_met"] = bool(dE <= 1e-3)
         rec["error_windows"] = {k: {"max_abs_dE_cm1": float(f"{v[0]:.3e}"), "roots": v[1]} for k, v in win.items()}
-        rec["error_note"] = ("max_abs_dE_cm1: FP64 roots converged (bji < 1e-9 hartree) within 0-6000 cm^-1 above "
-                             "the lowest, each matched to the nearest mixed-run root; no re-orthogonalisation, so "
-                             "converged roots have ghost copies in both runs and intensities are not compared")
+        rec["error_note"] = (
+            "max_abs_dE_cm1: roots within 0-6000 cm^-1 above the lowest converged (bji < 1e-9 hartree) in both the "
+            "FP64 and the mixed run, each FP64 root matched to the nearest mixed-run root.  n_converged_fp64 - "
+            "n_converged_both window roots converged in FP64 are still converging in the mixed run after the same "
+            "iterations (error_windows.0_6000 includes them: their gap is the Ritz residual, not a converged shift).  "
+            "No re-orthogonalisation, so converged roots have ghost copies in both runs and intensities are not "
+            "compared")
     if not args.no_cfg2:
         rec["reorthogonalised_1p38M"] = cfg2_jobs(args, sync)
     if not args.no_batch:

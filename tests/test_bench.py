@@ -49,5 +49,18 @@ def test_compare_windows_separates_window_all_and_both_converged():
     bj = np.array([1e-12, 1e-12, 1e-12, 1e-4, 1e-12])
     w = bench.compare_windows(ev64, bj64, ev, bj)
     assert w["0_6000"][1] == 3 and abs(w["0_6000"][0] - 1e-4) < 1e-7
+    assert w["0_6000_both"][1] == 3 and abs(w["0_6000_both"][0] - 1e-4) < 1e-7
     assert w["all"][1] == 4 and abs(w["all"][0] - 0.5) < 1e-7
     assert w["both"][1] == 3 and abs(w["both"][0] - 1e-4) < 1e-7
+
+
+def test_compare_windows_leaves_roots_still_converging_out_of_the_both_figure():
+    au = bench.AU2WAVE
+    ev64 = np.array([0.0, 1000.0, 3000.0]) / au
+    bj64 = np.full(3, 1e-12)
+    # the 3000 cm^-1 root is still converging in the mixed run: off by 0.02 with residual 1e-7
+    ev = np.array([0.0, 1000.00002, 3000.02]) / au
+    bj = np.array([1e-12, 1e-12, 1e-7])
+    w = bench.compare_windows(ev64, bj64, ev, bj)
+    assert w["0_6000"][1] == 3 and abs(w["0_6000"][0] - 0.02) < 1e-7
+    assert w["0_6000_both"][1] == 2 and abs(w["0_6000_both"][0] - 2e-5) < 1e-7
