@@ -244,6 +244,65 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
             h.on = e == hipSuccess ? 1 : 0;
         }
     }
+    // Fast-block H*v: by default from 16 MB of interleaved vector, where the
+    // vector leaves the caches (four states, ordr 3, 2.1M rows: FP32 633 against
+    // 3286 us for the run-time compiled kernel, FP64 1578 against 1916 us; 22M
+    // rows: 13.2 against 17.05 ms and 30.3 against 38.0 ms for the two-pass
+    // form); below, the cache-resident vector favours the other kernels (three
+    // states, 0.46M rows: 40.9 against 26.0 us).  NV_HV_FB=1 takes it at any
+    // size, NV_HV_FB=0 turns it off.
+    d.fb = NvFbDev{};
+    d.fb_min_bytes = (int64_t)16 << 20;
+    {
+        const char *k = std::getenv("NV_HV_FB");
+        if (k && std::strcmp(k, "1") == 0) d.fb_min_bytes = 0;
+        if (op.fb && !(k && std::strcmp(k, "0") == 0)) {
+            NvFbDev &f = d.fb;
+            f.mf = op.fb_mf;
+            f.P = op.fb_P;
+            f.nblk = (int32_t)(op.D / op.fb_P);
+            f.halo = op.fb_halo;
+            f.pad = op.fb_pad;
+            f.nch = op.fb_nch;
+            for (int q = 0; q <= NV_FB_NSLOW; ++q) f.sgrp[q] = op.fb_sgrp[q];
+            for (int q = 0; q <= NV_FB_NCH; ++q) f.chs[q] = op.fb_chs[q];
+            f.npure = (int32_t)op.fb_pure.size();
+            f.ntab = (int32_t)op.fb_tabmode.size();
+            // sweep order: rows in order (one digit of nblk blocks of P rows), or NV_FB_ORDER, a
+            // permutation of the slow modes listed from the fastest-varying in the sweep
+            f.nord = 1;
+            f.ordnf[0] = f.nblk;
+            f.ordshft[0] = f.P;
+            if (const char *o = std::getenv("NV_FB_ORDER")) {
+                std::vector<int> ms;
+                for (const char *q = o; *q;) {
+                    char *end = nullptr;
+                    const long v = std::strtol(q, &end, 10);
+                    if (end == q) break;
+                    ms.push_back((int)v);
+                    q = *end == ',' ? end + 1 : end;
+                }
+                std::vector<int> seen(op.fb_mf, 0);
+                bool ok = (int)ms.size() == op.fb_mf && op.fb_mf <= 16;
+                for (int m : ms) ok = ok && m >= 0 && m < op.fb_mf && !seen[m]++;
+                if (ok) {
+                    f.nord = op.fb_mf;
+                    for (int j = 0; j < op.fb_mf; ++j) {
+                        f.ordnf[j] = (int32_t)op.nfunc[ms[j]];
+                        f.ordshft[j] = (int32_t)op.shft[ms[j]];
+                    }
+                }
+            }
+            std::vector<float> t32(op.fb_tab.begin(), op.fb_tab.end());
+            if (e == hipSuccess) e = upload(c, op.fb_S, &f.S, c->allocs);
+            if (e == hipSuccess) e = upload(c, op.fb_steps, &f.steps, c->allocs);
+            if (e == hipSuccess && !op.fb_pure.empty()) e = upload(c, op.fb_pure, &f.pure, c->allocs);
+            if (e == hipSuccess) e = upload(c, op.fb_tab, &f.tab64, c->allocs);
+            if (e == hipSuccess) e = upload(c, op.fb_tabmode, &f.tabmode, c->allocs);
+            if (e == hipSuccess) e = upload(c, t32, &f.tab32, c->allocs);
+            f.on = e == hipSuccess ? 1 : 0;
+        }
+    }
     {
         const char *k = std::getenv("NV_HV_KERNEL");
         if (k && std::strcmp(k, "steps") == 0) d.tshape = nullptr;
@@ -309,8 +368,6 @@ int32_t nv_destroy(nv_ctx *c) {
     release_lanczos(c);
     for (auto &k : c->jit) nv::jit_release(k);
     free_list(c->gallocs);
-    if (c->hx32) hipFree(c->hx32);
-    if (c->hx64i) hipFree(c->hx64i);
     if (c->z2) hipFree(c->z2);
     free_list(c->allocs);
     for (auto &p : c->ev) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
@@ -351,8 +408,8 @@ static const nv::JitKernels *ensure_jit(nv_ctx *c, int32_t prec) {
     // faster one (three states, N = 1.38M: 49.7 against 58.6 us); the specialised FP64 kernel serves
     // the operators it does not take
     if (prec == NV_PREC_F64 && nv::hv_shaped(c->dev, NV_PREC_F64)) return nullptr;
-    // large vectors: the two-pass H*v
-    if (c->dev.tshape && nv::hv2_use(c->dev, prec)) return nullptr;
+    // large vectors: the fast-block or the two-pass H*v
+    if (c->dev.tshape && (nv::hv_fb_use(c->dev, prec) || nv::hv2_use(c->dev, prec))) return nullptr;
     nv::JitKernels &k = c->jit[prec];
     if (k.ready) return &k;
     if (k.failed) return nullptr;
@@ -395,34 +452,69 @@ static hipError_t ensure_z2(nv_ctx *c, size_t bytes) {
     return e;
 }
 
-// Whether the two-pass H*v serves precision prec on this context (the
+// Rows of zero margin before and after every interleaved copy of a vector:
+// the fast-block H*v reads up to op.fb_pad rows beyond either end.
+static size_t il_pad(const nv_ctx *c) { return c->op.fb ? (size_t)c->op.fb_pad : 0; }
+
+// An interleaved copy (D rows of nv_lo_stride(ns) values of el bytes) with
+// its margins, zeroed on stream s; *first is its first row and the
+// allocation joins owner.
+static hipError_t alloc_interleaved(nv_ctx *c, size_t el, std::vector<void *> &owner, void **first, hipStream_t s) {
+    const size_t nsp = (size_t)nv_lo_stride((int)c->op.ns), pad = il_pad(c);
+    const size_t bytes = ((size_t)c->op.D + 2 * pad) * nsp * el;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return e;
+    owner.push_back(p);
+    e = hipMemsetAsync(p, 0, bytes, s);
+    *first = (char *)p + pad * nsp * el;
+    return e;
+}
+
+// Whether the fast-block H*v serves precision prec on this context (the
 // shaped kernels selected and the tables built for a large enough vector).
-static bool use_hv2(const nv_ctx *c, int32_t prec) { return c->dev.tshape && nv::hv2_use(c->dev, prec); }
+static bool use_fb(const nv_ctx *c, int32_t prec) { return c->dev.tshape && nv::hv_fb_use(c->dev, prec); }
+
+// Whether the two-pass H*v serves precision prec (when the fast-block form does not).
+static bool use_hv2(const nv_ctx *c, int32_t prec) {
+    return c->dev.tshape && !use_fb(c, prec) && nv::hv2_use(c->dev, prec);
+}
+
+// The interleaved copy of x that the fast-block and two-pass forms read:
+// FP32 for prec NV_PREC_F32, FP64 otherwise.
+static hipError_t interleaved_input(nv_ctx *c, int32_t prec, const double *x, const void **xin, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if (prec == NV_PREC_F32) {
+        if (!c->hx32) {
+            e = alloc_interleaved(c, sizeof(float), c->allocs, (void **)&c->hx32, s);
+            if (e != hipSuccess) return e;
+        }
+        e = nv::launch_cast_f32(x, c->hx32, c->op.N, c->op.D, (int)c->op.ns, s);
+        *xin = c->hx32;
+    } else {
+        if (!c->hx64i) {
+            e = alloc_interleaved(c, sizeof(double), c->allocs, (void **)&c->hx64i, s);
+            if (e != hipSuccess) return e;
+        }
+        e = nv::launch_cast_f64i(x, c->hx64i, c->op.N, c->op.D, (int)c->op.ns, s);
+        *xin = c->hx64i;
+    }
+    return e;
+}
 
 // y = H x (double in and out) on stream s, couplings in precision prec.
 static hipError_t hv_apply(nv_ctx *c, int32_t prec, const double *x, double *y, hipStream_t s) {
+    if (use_fb(c, prec)) {
+        const void *xin = nullptr;
+        hipError_t e = interleaved_input(c, prec, x, &xin, s);
+        if (e != hipSuccess) return e;
+        return nv::launch_hv_fb_full(c->dev, prec, x, xin, y, s);
+    }
     if (use_hv2(c, prec)) {
-        const size_t il = (size_t)c->op.D * nv_lo_stride((int)c->op.ns);
         hipError_t e = ensure_z2(c, nv::hv2_scratch_bytes(c->dev, prec));
         if (e != hipSuccess) return e;
         const void *xin = nullptr;
-        if (prec == NV_PREC_F32) {
-            if (!c->hx32) {
-                e = hipMalloc(&c->hx32, il * sizeof(float));
-                if (e == hipSuccess) e = hipMemsetAsync(c->hx32, 0, il * sizeof(float), s);
-                if (e != hipSuccess) return e;
-            }
-            e = nv::launch_cast_f32(x, c->hx32, c->op.N, c->op.D, (int)c->op.ns, s);
-            xin = c->hx32;
-        } else {
-            if (!c->hx64i) {
-                e = hipMalloc(&c->hx64i, il * sizeof(double));
-                if (e == hipSuccess) e = hipMemsetAsync(c->hx64i, 0, il * sizeof(double), s);
-                if (e != hipSuccess) return e;
-            }
-            e = nv::launch_cast_f64i(x, c->hx64i, c->op.N, c->op.D, (int)c->op.ns, s);
-            xin = c->hx64i;
-        }
+        e = interleaved_input(c, prec, x, &xin, s);
         if (e != hipSuccess) return e;
         return nv::launch_hv2_full(c->dev, prec, x, xin, c->z2, y, s);
     }
@@ -431,9 +523,7 @@ static hipError_t hv_apply(nv_ctx *c, int32_t prec, const double *x, double *y, 
     const void *xin = x;
     if (prec == NV_PREC_F32) {
         if (!c->hx32) {
-            const size_t bytes = (size_t)c->op.D * nv_lo_stride((int)c->op.ns) * sizeof(float);
-            hipError_t e = hipMalloc(&c->hx32, bytes);
-            if (e == hipSuccess) e = hipMemsetAsync(c->hx32, 0, bytes, s);
+            hipError_t e = alloc_interleaved(c, sizeof(float), c->allocs, (void **)&c->hx32, s);
             if (e != hipSuccess) return e;
         }
         hipError_t e = nv::launch_cast_f32(x, c->hx32, c->op.N, c->op.D, (int)c->op.ns, s);
@@ -575,9 +665,7 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
         NV_HIP(c, alloc((size_t)st.nslots * N * sizeof(double), (void **)&st.V));
         const size_t el = o->prec == NV_PREC_F64 ? 8 : (o->prec == NV_PREC_F32 ? 4 : 2);
         if (o->prec != NV_PREC_F64) {
-            const size_t lo_bytes = (size_t)c->op.D * nv_lo_stride((int)c->op.ns) * el;
-            NV_HIP(c, alloc(lo_bytes, &st.vlo));
-            NV_HIP(c, hipMemsetAsync(st.vlo, 0, lo_bytes, c->stream));
+            NV_HIP(c, alloc_interleaved(c, el, c->lz_allocs, &st.vlo, c->stream));
             // re-orthogonalisation reads the FP32 copy of the archive (half the bytes)
             if (o->reorthog > 0) {
                 st.ldf = N + (N & 1);
@@ -592,6 +680,7 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
         // two-pass H*v one per workgroup of its second pass
         int64_t nparts = std::max<int64_t>(nv::hv_blocks(c->dev), 8192) + 64;
         if (c->dev.hv2.on) nparts = std::max<int64_t>(nparts, nv::hv2_parts(c->dev) + 64);
+        if (c->dev.fb.on) nparts = std::max<int64_t>(nparts, nv::hv_fb_parts(c->dev) + 64);
         NV_HIP(c, alloc(nparts * sizeof(double), (void **)&st.partials));
         NV_HIP(c, alloc(nv::kNumCounters * sizeof(unsigned), (void **)&st.counters));
         NV_HIP(c, alloc(sizeof(int64_t), (void **)&c->d_iter));
@@ -616,14 +705,7 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
     const bool jit = ensure_jit(c, o->prec) != nullptr;
     st.v64i = nullptr;
     if (o->prec == NV_PREC_F64 && !jit && nv::hv_shaped(c->dev, NV_PREC_F64)) {
-        if (!c->v64i) {
-            const size_t bytes = (size_t)c->op.D * nv_lo_stride((int)c->op.ns) * sizeof(double);
-            void *p = nullptr;
-            NV_HIP(c, hipMalloc(&p, bytes));
-            c->allocs.push_back(p);
-            c->v64i = (double *)p;
-            NV_HIP(c, hipMemsetAsync(p, 0, bytes, c->stream));
-        }
+        if (!c->v64i) NV_HIP(c, alloc_interleaved(c, sizeof(double), c->allocs, (void **)&c->v64i, c->stream));
         st.v64i = c->v64i;
     }
     st.z2 = nullptr;
@@ -695,7 +777,14 @@ int32_t nv_hv_kernel_info(nv_ctx *c, int32_t prec, char *buf, int64_t len) {
     std::string t = k ? "operator-specialised (hiprtc)"
                       : (std::string(nv::hv_shaped(c->dev, prec) ? "generic shaped" : "generic step-table") +
                          (c->jit_log.empty() ? std::string() : ": " + c->jit_log));
-    if (!k && use_hv2(c, prec))
+    if (!k && use_fb(c, prec)) {
+        char fbs[160];
+        std::snprintf(fbs, sizeof fbs,
+                      "generic fast-block (%d rows per wave over modes %d..%lld; slow steps scalar, fast steps "
+                      "through LDS)",
+                      (int)c->dev.fb.P, (int)c->dev.fb.mf, (long long)c->op.M - 1);
+        t = fbs;
+    } else if (!k && use_hv2(c, prec))
         t = std::string("generic two-pass (modes 0 and 1 split off; coupling products on ") +
             (prec == NV_PREC_F32 && c->dev.hv2.mfma ? "MFMA" : "the VALU") + ")";
     else if (!k && nv::hv_shaped(c->dev, prec)) {

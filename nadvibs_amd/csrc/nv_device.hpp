@@ -99,6 +99,63 @@ struct NvHv2Dev {
     NvSigma sig[NV_HV2_NG];
 };
 
+// Fast-block H*v.  The trailing modes mf..M-1 ("fast") span P <= 64 rows, so
+// a wave of 64 lanes holds one block of P consecutive rows (lane p = row
+// row0 + p) whose slow digits n_0..n_{mf-1} are the same for every lane.  A
+// term t = C_t (x) P_S (x) P_F splits into its slow monomial S and its fast
+// monomial F; every distinct S is gathered once per wave,
+//   w_S[p] = sum_{Delta_s} f_S(Delta_s) x[row0 + Delta_s + p],
+// with wave-uniform factors and offsets (an out-of-basis slow step has factor
+// 0 and reads the zero margin of pad rows before the interleaved copy of x),
+// and mixed into one channel per fast monomial, V_F += C_t w_S.  Channel 0
+// (F = 1) is y itself; each other channel is applied once through LDS,
+//   y[p] += sum_{Delta_f} h_F(p, Delta_f) V_F[p + Delta_f].
+// Terms with no slow step whose fast monomial is no channel, or whose
+// Delta = 0 step is diagonal, are applied one by one from x[row0 + p].
+// Slow shapes (stepping slots, slowest first): (2) (3) (4) (2,2) (3,2) (2,3)
+// (2,2,2) and () (no stepping slow slot).
+#define NV_FB_NSLOW 8
+#define NV_FB_NCH 10       // channels, F = 1 included
+struct NvFbS {
+    int32_t base;          // row offset of the first slow step, -sum_j s_j shft_j
+    int32_t inc[3];        // 2 shft_j of the stepping slow slots
+    int32_t tab[3];        // their factor rows: tab[(row * nmax + n) * 4 + i] = h(p, n, -s + 2i), 0 outside the basis
+    int32_t nd;            // slow slots whose only step is Delta = 0 (at most 2)
+    int32_t dtab[2];       // their factor rows; entry 0 is h(p, n, 0)
+    int32_t cm[NV_FB_NCH]; // per channel: coupling matrix (row of cmat) of the term (S, F), or -1
+    int32_t excl0;         // channel 0 leaves out the middle step Delta_s = 0 (all stepping slots even)
+    int32_t chmask;        // bit c: cm[c] >= 0
+};
+struct NvFbPure {
+    int32_t cm;            // coupling matrix
+    int32_t nd;            // slow factors at Delta = 0
+    int32_t dtab[2];
+    int32_t s0, s1;        // its fast steps [s0, s1) of the step list
+};
+struct NvFbDev {
+    int32_t on;            // tables valid for this operator
+    int32_t mf;            // first fast mode
+    int32_t P;             // rows per block, prod_{k >= mf} nfunc_k (33..64)
+    int32_t nblk;          // D / P
+    int32_t halo;          // largest |lane offset| of a fast step
+    int32_t pad;           // zero rows before and after the interleaved copy of x (>= 64, and
+                           // the largest |row offset| of a slow step)
+    int32_t nord;          // slow modes in the order blocks are swept, the first varying fastest:
+    int32_t ordnf[16];     //   block t of the sweep decodes to digits of these sizes
+    int32_t ordshft[16];   //   whose rows are these strides (row0 = sum digit * stride)
+    int32_t ntab;          // factor rows
+    const int32_t *tabmode;  // [ntab] mode of each factor row
+    const double *tab64;   // [ntab][nmax][4]
+    const float *tab32;
+    const NvFbS *S;        // grouped by slow shape
+    int32_t sgrp[NV_FB_NSLOW + 1];
+    int32_t nch;           // channels in use
+    int32_t chs[NV_FB_NCH + 1];  // fast steps of channel c: [chs[c], chs[c+1]) (c >= 1)
+    const NvStep *steps;   // fast steps: stride = lane offset; factors (mode, ftab row)
+    int32_t npure;
+    const NvFbPure *pure;
+};
+
 // Everything the H*v kernels read, passed by value.
 struct NvOpDev {
     int32_t ns, M, D, N, nmax, ncodes, nterms, ndterms;
@@ -120,4 +177,6 @@ struct NvOpDev {
     int64_t skip_min_bytes;         // shaped H*v: vector size above which out-of-basis loads are skipped
     NvHv2Dev hv2;                   // two-pass H*v tables (hv2.on = 0: not available)
     int64_t hv2_min_bytes;          // vector size from which the two-pass H*v serves the iteration
+    NvFbDev fb;                     // fast-block H*v tables (fb.on = 0: not available)
+    int64_t fb_min_bytes;           // vector size from which the fast-block H*v serves
 };

@@ -311,28 +311,31 @@ template <int NS> struct XRes<NS, float> {
     __device__ __forceinline__ XRes(const float *x, int D)
         : r(__builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)((unsigned)D * kEl), kBufFlags)) {}
     template <typename TAcc>
-    __device__ __forceinline__ void fma(unsigned o, TAcc h, TAcc (&w)[NS]) const {
+    __device__ __forceinline__ void fma(unsigned o, TAcc h, TAcc (&w)[NS]) const { fma_so<TAcc>(o, 0u, h, w); }
+    // offset o (per lane) plus so (wave-uniform, a scalar register)
+    template <typename TAcc>
+    __device__ __forceinline__ void fma_so(unsigned o, unsigned so, TAcc h, TAcc (&w)[NS]) const {
         constexpr int P = nv_lo_stride(NS);
         if constexpr (P == 1) {
-            const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)o, 0, 0));
+            const float v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)o, (int)so, 0));
             w[0] += h * (TAcc)v;
         } else if constexpr (P == 2) {
-            const nv_f2 v = __builtin_bit_cast(nv_f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o, 0, 0));
+            const nv_f2 v = __builtin_bit_cast(nv_f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o, (int)so, 0));
 #pragma unroll
             for (int s = 0; s < NS; ++s) w[s] += h * (TAcc)v[s];
         } else if constexpr (P == 3) {
-            const nv_f3 v = __builtin_bit_cast(nv_f3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)o, 0, 0));
+            const nv_f3 v = __builtin_bit_cast(nv_f3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)o, (int)so, 0));
 #pragma unroll
             for (int s = 0; s < NS; ++s) w[s] += h * (TAcc)v[s];
         } else if constexpr (P == 4) {
-            const nv_f4 v = __builtin_bit_cast(nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)o, 0, 0));
+            const nv_f4 v = __builtin_bit_cast(nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)o, (int)so, 0));
 #pragma unroll
             for (int s = 0; s < NS; ++s) w[s] += h * (TAcc)v[s];
         } else {
 #pragma unroll
             for (int part = 0; part * 4 < NS; ++part) {
                 const nv_f4 v = __builtin_bit_cast(
-                    nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(o + 16u * part), 0, 0));
+                    nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(o + 16u * part), (int)so, 0));
 #pragma unroll
                 for (int s = part * 4; s < NS && s < part * 4 + 4; ++s) w[s] += h * (TAcc)v[s - part * 4];
             }
@@ -368,18 +371,20 @@ template <int NS> struct XRes<NS, F64I> {
     __device__ __forceinline__ XRes(const double *x, int D)
         : r(__builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)((unsigned)D * kEl), kBufFlags)) {}
     template <typename TAcc>
-    __device__ __forceinline__ void fma(unsigned o, TAcc h, TAcc (&w)[NS]) const {
+    __device__ __forceinline__ void fma(unsigned o, TAcc h, TAcc (&w)[NS]) const { fma_so<TAcc>(o, 0u, h, w); }
+    template <typename TAcc>
+    __device__ __forceinline__ void fma_so(unsigned o, unsigned so, TAcc h, TAcc (&w)[NS]) const {
         typedef double nv_d2 __attribute__((ext_vector_type(2)));
 #pragma unroll
         for (int part = 0; 2 * part < NS; ++part) {
             if (2 * part + 1 < NS) {
                 const nv_d2 v = __builtin_bit_cast(
-                    nv_d2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(o + 16u * part), 0, 0));
+                    nv_d2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(o + 16u * part), (int)so, 0));
                 w[2 * part] += h * (TAcc)v[0];
                 w[2 * part + 1] += h * (TAcc)v[1];
             } else {
                 const double v = __builtin_bit_cast(
-                    double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(o + 16u * part), 0, 0));
+                    double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(o + 16u * part), (int)so, 0));
                 w[2 * part] += h * (TAcc)v;
             }
         }
@@ -1064,6 +1069,355 @@ __global__ __launch_bounds__(kBlock) void hv2_pass2_kernel(NvOpDev op, LzState s
     }
 }
 
+// ------------------------------------------------------------------------
+// Fast-block H*v (tables in NvFbDev).  A wave owns one block of P <= 64
+// consecutive rows, lane p = row row0 + p: the trailing ("fast") modes vary
+// across the lanes and the slow digits are the same for the whole wave.
+//   1. every distinct slow monomial S: w_S[p] = sum f_S(Delta_s) x[row0 +
+//      Delta_s + p] (wave-uniform factors from a per-wave LDS table, offsets
+//      in the scalar offset field of the buffer loads), mixed into the
+//      channels V_F += C_t w_S of the fast monomials it pairs with (V_1 = y);
+//   2. every channel F > 1: y[p] += sum h_F(p, Delta_f) V_F[p + Delta_f]
+//      through a wave-private LDS row with zero margins;
+//   3. the remaining terms without slow steps, one by one from x[row0 + p].
+// An out-of-basis slow step has factor 0 and reads the first rows of the
+// zero margin that the interleaved copy of x carries, which all such steps
+// share; out-of-block fast neighbours carry factor 0.
+// ------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ const T *fbtab_of(const NvOpDev &op);
+template <> __device__ __forceinline__ const double *fbtab_of<double>(const NvOpDev &op) { return op.fb.tab64; }
+template <> __device__ __forceinline__ const float *fbtab_of<float>(const NvOpDev &op) { return op.fb.tab32; }
+
+// The four factors h(p, n, -s + 2i) of a slow slot at the wave's digit.
+template <typename T> struct Fb4 { T v[4]; };
+
+// The slow monomials [t0, t1) of shape (K0, K1, K2), gathered and mixed into
+// the channels V; vl is the lane's byte offset in its block, rb the byte
+// offset of the block's first row in the padded buffer, fw/fwm the wave's
+// factor rows and their masks of slots inside the basis.
+template <int NS, typename TIn, typename TAcc, int K0, int K1, int K2>
+__device__ __forceinline__ void fb_slow(const NvOpDev &op, const XRes<NS, TIn> &xs, unsigned vl, unsigned rb,
+                                        const TAcc *fw, const int *fwm, int t0, int t1,
+                                        TAcc (&V)[NV_FB_NCH][NS]) {
+    constexpr unsigned kEl = XRes<NS, TIn>::kEl;
+    constexpr int J0 = K0 ? K0 : 1, J1 = K1 ? K1 : 1, J2 = K2 ? K2 : 1;
+    // all stepping slots odd: the middle step is Delta_s = 0
+    constexpr bool odd = K0 > 0 && (K0 & 1) && (K1 == 0 || (K1 & 1)) && (K2 == 0 || (K2 & 1));
+    const TAcc *__restrict__ cm = cmat_of<TAcc>(op);
+    for (int t = t0; t < t1; ++t) {
+        const NvFbS d = op.fb.S[t];
+        Fb4<TAcc> f0, f1, f2;
+        // slot i of slow slot j is inside the basis when bit i of m_j is set
+        int m0 = 1, m1 = 1, m2 = 1;
+        if constexpr (K0 > 0) {
+            f0 = *reinterpret_cast<const Fb4<TAcc> *>(fw + d.tab[0] * 4);
+            m0 = __builtin_amdgcn_readfirstlane(fwm[d.tab[0]]);
+        }
+        if constexpr (K1 > 0) {
+            f1 = *reinterpret_cast<const Fb4<TAcc> *>(fw + d.tab[1] * 4);
+            m1 = __builtin_amdgcn_readfirstlane(fwm[d.tab[1]]);
+        }
+        if constexpr (K2 > 0) {
+            f2 = *reinterpret_cast<const Fb4<TAcc> *>(fw + d.tab[2] * 4);
+            m2 = __builtin_amdgcn_readfirstlane(fwm[d.tab[2]]);
+        }
+        const unsigned b0 = rb + (unsigned)d.base * kEl;
+        // w: every step but the middle one of an all-odd shape, wc: that one
+        TAcc w[NS], wc[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) w[s] = wc[s] = (TAcc)0;
+#pragma unroll
+        for (int i0 = 0; i0 < J0; ++i0)
+#pragma unroll
+            for (int i1 = 0; i1 < J1; ++i1)
+#pragma unroll
+                for (int i2 = 0; i2 < J2; ++i2) {
+                    TAcc h = (TAcc)1;
+                    unsigned so = rb;
+                    if constexpr (K0 > 0) {
+                        h = f0.v[i0];
+                        so = b0 + (unsigned)(i0 * d.inc[0]) * kEl;
+                    }
+                    if constexpr (K1 > 0) {
+                        h *= f1.v[i1];
+                        so += (unsigned)(i1 * d.inc[1]) * kEl;
+                    }
+                    if constexpr (K2 > 0) {
+                        h *= f2.v[i2];
+                        so += (unsigned)(i2 * d.inc[2]) * kEl;
+                    }
+                    const bool in = ((m0 >> i0) & (K1 > 0 ? m1 >> i1 : 1) & (K2 > 0 ? m2 >> i2 : 1) & 1) != 0;
+                    const bool mid =
+                        odd && 2 * i0 + 1 == K0 && (K1 == 0 || 2 * i1 + 1 == K1) && (K2 == 0 || 2 * i2 + 1 == K2);
+                    if (mid) xs.template fma_so<TAcc>(vl, in ? so : 0u, h, wc);
+                    else xs.template fma_so<TAcc>(vl, in ? so : 0u, h, w);
+                }
+        if (d.nd > 0) {
+            TAcc fd = fw[d.dtab[0] * 4];
+            if (d.nd > 1) fd *= fw[d.dtab[1] * 4];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                w[s] *= fd;
+                wc[s] *= fd;
+            }
+        }
+        TAcc wf[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) wf[s] = w[s] + wc[s];
+        for (int c = 0; c < NV_FB_NCH; ++c) {
+            if (d.cm[c] < 0) continue;
+            const TAcc *C = cm + (size_t)d.cm[c] * NS * NS;
+            const bool ex = c == 0 && odd && d.excl0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                TAcc a = V[c][s];
+#pragma unroll
+                for (int s2 = 0; s2 < NS; ++s2) a += C[s * NS + s2] * (ex ? w[s2] : wf[s2]);
+                V[c][s] = a;
+            }
+        }
+    }
+}
+
+// y += sum_{steps [s0, s1)} h(p) src[p + stride]: the fast steps of one
+// channel or term over the wave's LDS row vb, src published there first.
+template <int NS, typename TAcc>
+__device__ __forceinline__ void fb_fast(const NvOpDev &op, int s0, int s1, TAcc *vb, int lane, const uint8_t *dig,
+                                        const TAcc *ftab, int tid, const TAcc (&src)[NS], TAcc (&y)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) vb[lane * NS + s] = src[s];
+    __builtin_amdgcn_wave_barrier();
+    const int nmax = op.nmax;
+    for (int k = s0; k < s1; ++k) {
+        const NvStep st = op.fb.steps[k];
+        TAcc h = ftab[st.code[0] * nmax + dig[st.mode[0] * kBlock + tid]];
+#pragma unroll
+        for (int j = 1; j < NV_MAXF; ++j)
+            if (j < st.nf) h *= ftab[st.code[j] * nmax + dig[st.mode[j] * kBlock + tid]];
+        const TAcc *v = vb + (lane + st.stride) * NS;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) y[s] += h * v[s];
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The off-diagonal part of the wave's block.
+template <int NS, typename TIn, typename TAcc>
+__device__ __forceinline__ void fb_offdiag(const NvOpDev &op, const XRes<NS, TIn> &xs, unsigned vl, unsigned rb,
+                                           int lane, const uint8_t *dig, const TAcc *fw, const int *fwm,
+                                           const TAcc *ftab, int tid, TAcc *vb, TAcc (&y)[NS]) {
+    TAcc V[NV_FB_NCH][NS];
+#pragma unroll
+    for (int c = 0; c < NV_FB_NCH; ++c)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) V[c][s] = (TAcc)0;
+    const int *g = op.fb.sgrp;
+    fb_slow<NS, TIn, TAcc, 2, 0, 0>(op, xs, vl, rb, fw, fwm, g[0], g[1], V);
+    fb_slow<NS, TIn, TAcc, 3, 0, 0>(op, xs, vl, rb, fw, fwm, g[1], g[2], V);
+    fb_slow<NS, TIn, TAcc, 4, 0, 0>(op, xs, vl, rb, fw, fwm, g[2], g[3], V);
+    fb_slow<NS, TIn, TAcc, 2, 2, 0>(op, xs, vl, rb, fw, fwm, g[3], g[4], V);
+    fb_slow<NS, TIn, TAcc, 3, 2, 0>(op, xs, vl, rb, fw, fwm, g[4], g[5], V);
+    fb_slow<NS, TIn, TAcc, 2, 3, 0>(op, xs, vl, rb, fw, fwm, g[5], g[6], V);
+    fb_slow<NS, TIn, TAcc, 2, 2, 2>(op, xs, vl, rb, fw, fwm, g[6], g[7], V);
+    fb_slow<NS, TIn, TAcc, 0, 0, 0>(op, xs, vl, rb, fw, fwm, g[7], g[8], V);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) y[s] = V[0][s];
+#pragma unroll
+    for (int c = 1; c < NV_FB_NCH; ++c)
+        if (c < op.fb.nch) fb_fast<NS, TAcc>(op, op.fb.chs[c], op.fb.chs[c + 1], vb, lane, dig, ftab, tid, V[c], y);
+    if (op.fb.npure > 0) {
+        TAcc xr[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) xr[s] = (TAcc)0;
+        xs.template fma_so<TAcc>(vl, rb, (TAcc)1, xr);
+        const TAcc *__restrict__ cm = cmat_of<TAcc>(op);
+        for (int k = 0; k < op.fb.npure; ++k) {
+            const NvFbPure u = op.fb.pure[k];
+            TAcc fd = (TAcc)1;
+            if (u.nd > 0) fd = fw[u.dtab[0] * 4];
+            if (u.nd > 1) fd *= fw[u.dtab[1] * 4];
+            const TAcc *C = cm + (size_t)u.cm * NS * NS;
+            TAcc v[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                TAcc a = (TAcc)0;
+#pragma unroll
+                for (int s2 = 0; s2 < NS; ++s2) a += C[s * NS + s2] * xr[s2];
+                v[s] = fd * a;
+            }
+            fb_fast<NS, TAcc>(op, u.s0, u.s1, vb, lane, dig, ftab, tid, v, y);
+        }
+    }
+}
+
+// LDS of the fast-block kernels: factor table, the digits of one row per
+// thread, four wave rows of (64 + 2 halo) lanes x NS values, four wave
+// tables of ntab factor rows x 4 and four of ntab validity masks.
+__host__ __device__ inline size_t fb_lds_layout(const NvOpDev &op, size_t el, size_t *dig_off, size_t *vb_off,
+                                                size_t *fw_off) {
+    const size_t a = ((size_t)op.ncodes * op.nmax * el + 15) & ~(size_t)15;
+    const size_t b = a + (((size_t)op.M * kBlock + 15) & ~(size_t)15);
+    const size_t c = b + ((4 * (size_t)(64 + 2 * op.fb.halo) * op.ns * el + 15) & ~(size_t)15);
+    if (dig_off) *dig_off = a;
+    if (vb_off) *vb_off = b;
+    if (fw_off) *fw_off = c;
+    // factor rows, then one validity mask per row
+    return c + 4 * (size_t)op.fb.ntab * (4 * el + sizeof(int));
+}
+
+// Common prologue: factor table and digits in LDS, the wave's block, its LDS
+// row and its factor rows.
+template <typename TAcc> struct FbWave {
+    const TAcc *ftab;
+    uint8_t *dig;
+    TAcc *vb;
+    const TAcc *fw;
+    const int *fwm;
+    int lane, wave, blk, row0;
+    bool inblk, live;
+};
+template <typename TAcc> __device__ __forceinline__ FbWave<TAcc> fb_prologue(const NvOpDev &op, int b) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    FbWave<TAcc> w;
+    size_t dig_off, vb_off, fw_off;
+    fb_lds_layout(op, sizeof(TAcc), &dig_off, &vb_off, &fw_off);
+    TAcc *ft = reinterpret_cast<TAcc *>(lds_raw);
+    const int nt = op.ncodes * op.nmax;
+    const TAcc *g = ftab_of<TAcc>(op);
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) ft[i] = g[i];
+    w.ftab = ft;
+    w.dig = lds_raw + dig_off;
+    const int tid = threadIdx.x;
+    w.lane = tid & 63;
+    w.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    w.blk = b * 4 + w.wave;
+    w.inblk = w.blk < op.fb.nblk;
+    {
+        // block t of the sweep: its slow digits in the sweep's mode order
+        int t = w.blk, row0 = 0;
+        for (int j = 0; j < op.fb.nord; ++j) {
+            const int nf = op.fb.ordnf[j];
+            const int q = t / nf;
+            row0 += (t - q * nf) * op.fb.ordshft[j];
+            t = q;
+        }
+        w.row0 = row0;
+    }
+    w.live = w.inblk && w.lane < op.fb.P;
+    if (w.live) digits_energy(op, w.row0 + w.lane, w.dig, tid);
+    else for (int m = 0; m < op.M; ++m) w.dig[m * kBlock + tid] = 0;
+    const int rowlen = (64 + 2 * op.fb.halo) * op.ns;
+    TAcc *vrow = reinterpret_cast<TAcc *>(lds_raw + vb_off) + (size_t)w.wave * rowlen;
+    for (int i = w.lane; i < rowlen; i += 64) vrow[i] = (TAcc)0;
+    w.vb = vrow + (size_t)op.fb.halo * op.ns;
+    __syncthreads();
+    // the wave's factor rows at its slow digits (those of its first lane), and
+    // their masks of slots inside the basis (factor nonzero)
+    TAcc *fw = reinterpret_cast<TAcc *>(lds_raw + fw_off) + (size_t)w.wave * op.fb.ntab * 4;
+    int *fwm = reinterpret_cast<int *>(lds_raw + fw_off + 4 * (size_t)op.fb.ntab * 4 * sizeof(TAcc)) +
+               (size_t)w.wave * op.fb.ntab;
+    const TAcc *tab = fbtab_of<TAcc>(op);
+    for (int k = w.lane; k < op.fb.ntab; k += 64) {
+        const int n = w.dig[op.fb.tabmode[k] * kBlock + w.wave * 64];
+        int m = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const TAcc f = tab[((size_t)k * op.nmax + n) * 4 + i];
+            fw[k * 4 + i] = f;
+            m |= (f != (TAcc)0) << i;
+        }
+        fwm[k] = m;
+    }
+    __builtin_amdgcn_wave_barrier();
+    w.fw = fw;
+    w.fwm = fwm;
+    return w;
+}
+
+// Iteration H*v in the fast-block form: y = offdiag(H) q_i, alpha partials.
+// Waves per SIMD the compiler must fit (register budget): left free, the
+// FP32 kernel holds 87 VGPRs (5 waves) and the FP64 one 132 (3 waves); at 8
+// and 5 waves they run 13.2 and 30.3 ms against 15.1 and 36.2 ms (four
+// states, 22M rows; 7 and 4 waves: 13.8 and 30.5 ms).
+#ifndef NV_FB_WAVES32
+#define NV_FB_WAVES32 8
+#endif
+#ifndef NV_FB_WAVES64
+#define NV_FB_WAVES64 5
+#endif
+// (Above four states the ten channels of NS values need more registers: the
+// budget follows the state count.)
+constexpr int fb_waves(int ns, int bytes) {
+    return ns <= 4 ? (bytes == 8 ? NV_FB_WAVES64 : NV_FB_WAVES32) : (bytes == 8 ? 1 : 2);
+}
+template <int NS, typename TIn, typename TAcc>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(fb_waves(NS, sizeof(TAcc))))) void
+hv_fb_iter_kernel(NvOpDev op, LzState st) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const FbWave<TAcc> w = fb_prologue<TAcc>(op, b);
+    const int tid = threadIdx.x;
+    const int64_t it = *st.iter;
+    const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;
+    using TX = typename std::conditional<std::is_same<TIn, F64I>::value, double, TIn>::type;
+    const TX *x = std::is_same<TIn, F64I>::value ? reinterpret_cast<const TX *>(st.v64i)
+                                                 : reinterpret_cast<const TX *>(st.vlo);
+    constexpr int kNsp = nv_lo_stride(NS);
+    constexpr unsigned kEl = XRes<NS, TIn>::kEl;
+    const XRes<NS, TIn> xs(x - (size_t)op.fb.pad * kNsp, op.D + 2 * op.fb.pad);
+    TAcc y[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) y[s] = (TAcc)0;
+    // a lane past the block's P rows loads nothing (offset outside the buffer)
+    const unsigned vl = w.live ? (unsigned)w.lane * kEl : 0x80000000u;
+    const unsigned rb = (unsigned)(w.row0 + op.fb.pad) * kEl;
+    if (w.inblk) fb_offdiag<NS, TIn, TAcc>(op, xs, vl, rb, w.lane, w.dig, w.fw, w.fwm, w.ftab, tid, w.vb, y);
+    double part = 0.0;
+    if (w.live) {
+        const int r = w.row0 + w.lane;
+        double v[NS], dv[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
+        diag_apply<NS>(op, r, w.dig, tid, v, dv);
+        TAcc *yo = reinterpret_cast<TAcc *>(st.y);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            yo[(size_t)s * op.D + r] = y[s];
+            part += v[s] * (dv[s] + (double)y[s]);
+        }
+    }
+    const double bs = block_sum(part);
+    if (tid == 0) st.partials[b] = bs;
+}
+
+// Standalone y = H x in the fast-block form: xin is the interleaved copy of x
+// (FP32 or FP64, with its margins), x the FP64 vector for the diagonal part.
+template <int NS, typename TIn, typename TAcc>
+__global__ __launch_bounds__(kBlock) void hv_fb_full_kernel(NvOpDev op, const void *__restrict__ xin,
+                                                            const double *__restrict__ x, double *__restrict__ yfull) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const FbWave<TAcc> w = fb_prologue<TAcc>(op, b);
+    const int tid = threadIdx.x;
+    using TX = typename std::conditional<std::is_same<TIn, F64I>::value, double, TIn>::type;
+    constexpr int kNsp = nv_lo_stride(NS);
+    constexpr unsigned kEl = XRes<NS, TIn>::kEl;
+    const XRes<NS, TIn> xs(reinterpret_cast<const TX *>(xin) - (size_t)op.fb.pad * kNsp, op.D + 2 * op.fb.pad);
+    TAcc y[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) y[s] = (TAcc)0;
+    const unsigned vl = w.live ? (unsigned)w.lane * kEl : 0x80000000u;
+    const unsigned rb = (unsigned)(w.row0 + op.fb.pad) * kEl;
+    if (w.inblk) fb_offdiag<NS, TIn, TAcc>(op, xs, vl, rb, w.lane, w.dig, w.fw, w.fwm, w.ftab, tid, w.vb, y);
+    if (w.live) {
+        const int r = w.row0 + w.lane;
+        double v[NS], dv[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = x[(size_t)s * op.D + r];
+        diag_apply<NS>(op, r, w.dig, tid, v, dv);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) yfull[(size_t)s * op.D + r] = dv[s] + (double)y[s];
+    }
+}
+
 // lanczos_step (Source/top.f90:820-842): r = (H q_i - alpha q_i) - beta(i-1) q_{i-1}
 // stored in slot i+1, beta(i) = |r|.
 template <int NS, typename TAcc>
@@ -1420,6 +1774,55 @@ hipError_t launch_hv2_full(const NvOpDev &op, int prec, const double *x, const v
     NV_DISPATCH_NS(hv2_ns, op, prec, xin, Z, nullptr, x, y, s)
 }
 
+// ---- fast-block H*v launchers ----
+static size_t hv_fb_lds(const NvOpDev &op, int prec) {
+    return fb_lds_layout(op, prec == 0 ? 8 : 4, nullptr, nullptr, nullptr);
+}
+
+bool hv_fb_use(const NvOpDev &op, int prec) {
+    if (!op.tshape || !op.fb.on || (prec != 0 && prec != 1)) return false;
+    const int64_t el = (int64_t)nv_lo_stride(op.ns) * (prec == 0 ? 8 : 4);
+    return (int64_t)op.D * el >= op.fb_min_bytes && shaped_ok(op, el) && hv_fb_lds(op, prec) <= 64 * 1024 &&
+           ((int64_t)op.D + 2 * (int64_t)op.fb.pad) * el < ((int64_t)1 << 31);
+}
+
+int hv_fb_parts(const NvOpDev &op) { return (op.fb.nblk + 3) / 4; }
+
+template <int NS>
+static hipError_t fb_iter_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    const size_t lds = hv_fb_lds(op, prec);
+    const unsigned g = (unsigned)hv_fb_parts(op);
+    if (prec == 0)
+        hipLaunchKernelGGL((hv_fb_iter_kernel<NS, F64I, double>), dim3(g), dim3(kBlock), lds, s, op, st);
+    else
+        hipLaunchKernelGGL((hv_fb_iter_kernel<NS, float, float>), dim3(g), dim3(kBlock), lds, s, op, st);
+    return hipGetLastError();
+}
+
+template <int NS>
+static hipError_t fb_full_ns(const NvOpDev &op, int prec, const void *xin, const double *x, double *y,
+                             hipStream_t s) {
+    const size_t lds = hv_fb_lds(op, prec);
+    const unsigned g = (unsigned)hv_fb_parts(op);
+    if (prec == 0)
+        hipLaunchKernelGGL((hv_fb_full_kernel<NS, F64I, double>), dim3(g), dim3(kBlock), lds, s, op, xin, x, y);
+    else
+        hipLaunchKernelGGL((hv_fb_full_kernel<NS, float, float>), dim3(g), dim3(kBlock), lds, s, op, xin, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_hv_fb_full(const NvOpDev &op, int prec, const double *x, const void *xin, double *y,
+                             hipStream_t s) {
+    NV_DISPATCH_NS(fb_full_ns, op, prec, xin, x, y, s)
+}
+
+// The iteration H*v runs in the fast-block form when the operator has its
+// tables and the interleaved copy of q_i exists (FP32: LzState::vlo; FP64:
+// LzState::v64i).
+static bool fb_iter(const NvOpDev &op, int prec, const LzState &st) {
+    return hv_fb_use(op, prec) && (prec == 1 ? st.vlo != nullptr : st.v64i != nullptr);
+}
+
 // The iteration H*v runs two-pass when the caller gave the channel scratch
 // (LzState::z2) and, in FP64, the interleaved copy of q_i.
 static bool hv2_iter(const NvOpDev &op, int prec, const LzState &st) {
@@ -1427,6 +1830,7 @@ static bool hv2_iter(const NvOpDev &op, int prec, const LzState &st) {
 }
 
 static hipError_t hv_iter_any(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    if (fb_iter(op, prec, st)) NV_DISPATCH_NS(fb_iter_ns, op, prec, st, s)
     if (hv2_iter(op, prec, st)) {
         const void *xin = prec == 1 ? st.vlo : (const void *)st.v64i;
         NV_DISPATCH_NS(hv2_ns, op, prec, xin, st.z2, &st, nullptr, nullptr, s)
@@ -1435,6 +1839,7 @@ static hipError_t hv_iter_any(const NvOpDev &op, int prec, const LzState &st, hi
 }
 
 int hv_iter_parts(const NvOpDev &op, int prec, const LzState &st) {
+    if (fb_iter(op, prec, st)) return hv_fb_parts(op);
     return hv2_iter(op, prec, st) ? hv2_parts(op) : hv_blocks(op);
 }
 

@@ -56,6 +56,13 @@ int hv_iter_parts(const NvOpDev &op, int prec, const LzState &st);
 hipError_t launch_hv2_full(const NvOpDev &op, int prec, const double *x, const void *xin, void *Z, double *y,
                            hipStream_t s);
 hipError_t launch_cast_f64i(const double *x, double *y, int64_t n, int64_t D, int ns, hipStream_t s);
+// Fast-block H*v (NvFbDev): whether it serves precision prec (tables built,
+// vector at least op.fb_min_bytes), the number of alpha partials its
+// iteration kernel writes, and y = H x with xin the interleaved copy of x.
+bool hv_fb_use(const NvOpDev &op, int prec);
+int hv_fb_parts(const NvOpDev &op);
+hipError_t launch_hv_fb_full(const NvOpDev &op, int prec, const double *x, const void *xin, double *y,
+                             hipStream_t s);
 // The same for the batched H*v (which reads the state-major FP64 vector).
 bool hv_shaped_batch(const NvOpDev &op, int prec);
 // Default of NvOpDev::skip_min_bytes (128 MB).

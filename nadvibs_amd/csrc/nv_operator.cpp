@@ -2,6 +2,7 @@
 #include "nv_operator.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <map>
 #include <tuple>
@@ -217,6 +218,184 @@ static void build_two_pass(Operator &op, CodeOf &code_of) {
     }
     op.hv2_pgrp[NV_HV2_NSHAPES] = (int32_t)op.hv2_phi.size();
     op.hv2 = true;
+}
+
+// Tables of the fast-block H*v (NvFbDev).  The fast modes are the trailing
+// modes whose basis product P is at most 64 (one block of P rows per wave);
+// the form needs 33 <= P <= 64, a slow mode, ordr <= 3, at most NV_FB_NCH
+// channels and a known shape for every slow part.  fb_tab is filled from
+// ftab once the codes are final: tabcodes[k] lists the ftab rows of
+// Delta = -s + 2i of factor row k.
+template <typename CodeOf>
+static void build_fast_block(Operator &op, CodeOf &code_of, std::vector<std::array<int, 4>> &tabcodes) {
+    // slow shapes of NvFbS, in NV_FB_NSLOW order
+    static const int kSlow[NV_FB_NSLOW - 1][3] = {{2, 0, 0}, {3, 0, 0}, {4, 0, 0}, {2, 2, 0},
+                                                  {3, 2, 0}, {2, 3, 0}, {2, 2, 2}};
+    op.fb = false;
+    op.fb_S.clear();
+    op.fb_steps.clear();
+    op.fb_pure.clear();
+    op.fb_tabmode.clear();
+    tabcodes.clear();
+    if (!op.shaped || op.ordr > 3 || op.M < 2) return;
+    int mf = (int)op.M;
+    int64_t P = 1;
+    // as few fast modes as give 33..64 rows (the channels grow with them)
+    while (mf > 0 && P < 33 && P * op.nfunc[mf - 1] <= 64) P *= op.nfunc[--mf];
+    if (mf < 1 || P < 33) return;
+    std::map<std::pair<int, int>, int> tabs;  // (mode, pow) -> factor row
+    auto tab_of = [&](int m, int p, int s) {
+        auto key = std::make_pair(m, p);
+        auto it = tabs.find(key);
+        if (it != tabs.end()) return it->second;
+        std::array<int, 4> cs = {-1, -1, -1, -1};
+        for (int i = 0; i <= s; ++i) cs[i] = code_of(m, p, -s + 2 * i);
+        const int k = (int)tabcodes.size();
+        tabcodes.push_back(cs);
+        op.fb_tabmode.push_back(m);
+        tabs[key] = k;
+        return k;
+    };
+    typedef std::vector<std::pair<int, int>> Key;  // (mode, pow) of the slots
+    struct Split {
+        Key skey, fkey;
+        std::vector<int> sact, sdia, fslots;
+        bool pure;
+    };
+    std::vector<Split> sp(op.terms.size());
+    // channels: the fast monomials that meet a slow step (channel 0: no fast slot)
+    std::map<Key, int> chan;
+    chan[Key()] = 0;
+    std::vector<Key> chkeys(1);
+    std::vector<int> chterm(1, -1);  // a term that carries each channel's fast slots
+    for (size_t ti = 0; ti < op.terms.size(); ++ti) {
+        if (op.term_dev[ti] < 0) continue;
+        const Term &t = op.terms[ti];
+        Split &x = sp[ti];
+        for (int j = 0; j < t.cnt; ++j) {
+            if (t.mode[j] >= mf) {
+                x.fkey.push_back({t.mode[j], t.pow[j]});
+                x.fslots.push_back(j);
+            } else {
+                x.skey.push_back({t.mode[j], t.pow[j]});
+                (t.smax[j] > 0 ? x.sact : x.sdia).push_back(j);
+            }
+        }
+        x.pure = x.sact.empty();
+        if (x.sdia.size() > 2 || x.sact.size() > 3 || x.fslots.size() > NV_MAXF) return;
+        // a Delta = 0 step that is diagonal needs no slow step or no fast slot (else degree 4)
+        if (t.diag && !x.fkey.empty() && !x.pure) return;
+        if (!x.pure && !chan.count(x.fkey)) {
+            if ((int)chkeys.size() == NV_FB_NCH) return;
+            chan[x.fkey] = (int)chkeys.size();
+            chkeys.push_back(x.fkey);
+            chterm.push_back((int)ti);
+        }
+    }
+    // the fast steps of a term's fast slots (all of them; the Delta_f = 0 one left out when
+    // skip0), appended to fb_steps
+    int halo = 0;
+    auto fast_steps = [&](const Term &t, const std::vector<int> &fs, bool skip0) {
+        int d[NV_MAXF];
+        const int nfs = (int)fs.size();
+        for (int j = 0; j < nfs; ++j) d[j] = -t.smax[fs[j]];
+        while (true) {
+            bool zero = true;
+            int64_t off = 0;
+            for (int j = 0; j < nfs; ++j) {
+                off += (int64_t)d[j] * op.shft[t.mode[fs[j]]];
+                zero = zero && d[j] == 0;
+            }
+            if (!(skip0 && zero)) {
+                NvStep st;
+                st.stride = (int32_t)off;
+                st.nf = nfs;
+                for (int j = 0; j < NV_MAXF; ++j) { st.mode[j] = 0; st.code[j] = 0; }
+                for (int j = 0; j < nfs; ++j) {
+                    st.mode[j] = t.mode[fs[j]];
+                    st.code[j] = code_of(t.mode[fs[j]], t.pow[fs[j]], d[j]);
+                }
+                op.fb_steps.push_back(st);
+                halo = std::max<int>(halo, (int)std::abs(off));
+            }
+            int j = nfs - 1;
+            while (j >= 0 && d[j] >= t.smax[fs[j]]) { d[j] = -t.smax[fs[j]]; --j; }
+            if (j < 0) break;
+            d[j] += 2;
+        }
+    };
+    op.fb_chs[0] = op.fb_chs[1] = 0;
+    for (size_t c = 1; c < chkeys.size(); ++c) {
+        fast_steps(op.terms[chterm[c]], sp[chterm[c]].fslots, false);
+        op.fb_chs[c + 1] = (int32_t)op.fb_steps.size();
+    }
+    for (size_t c = chkeys.size(); c < NV_FB_NCH; ++c) op.fb_chs[c + 1] = op.fb_chs[c];
+    // slow monomials
+    std::vector<std::vector<NvFbS>> byshape(NV_FB_NSLOW);
+    std::map<Key, std::pair<int, int>> sindex;  // slow key -> (shape, position)
+    int64_t pad = 0;
+    for (size_t ti = 0; ti < op.terms.size(); ++ti) {
+        if (op.term_dev[ti] < 0) continue;
+        const Term &t = op.terms[ti];
+        const Split &x = sp[ti];
+        const auto ch = chan.find(x.fkey);
+        if (x.pure && (t.diag || ch == chan.end())) {
+            // applied on its own from x[row0 + p]
+            NvFbPure u{};
+            u.cm = op.term_dev[ti];
+            u.nd = (int32_t)x.sdia.size();
+            for (size_t j = 0; j < x.sdia.size(); ++j) u.dtab[j] = tab_of(t.mode[x.sdia[j]], t.pow[x.sdia[j]], 0);
+            u.s0 = (int32_t)op.fb_steps.size();
+            fast_steps(t, x.fslots, t.diag);
+            u.s1 = (int32_t)op.fb_steps.size();
+            op.fb_pure.push_back(u);
+            continue;
+        }
+        auto it = sindex.find(x.skey);
+        if (it == sindex.end()) {
+            NvFbS S{};
+            int64_t base = 0;
+            int k[3] = {0, 0, 0};
+            for (size_t j = 0; j < x.sact.size(); ++j) {
+                const int jj = x.sact[j], m = t.mode[jj];
+                base -= (int64_t)t.smax[jj] * op.shft[m];
+                S.inc[j] = (int32_t)(2 * op.shft[m]);
+                S.tab[j] = tab_of(m, t.pow[jj], t.smax[jj]);
+                k[j] = t.smax[jj] + 1;
+            }
+            S.base = (int32_t)base;
+            pad = std::max<int64_t>(pad, -base);
+            S.nd = (int32_t)x.sdia.size();
+            for (size_t j = 0; j < x.sdia.size(); ++j) S.dtab[j] = tab_of(t.mode[x.sdia[j]], t.pow[x.sdia[j]], 0);
+            for (int c = 0; c < NV_FB_NCH; ++c) S.cm[c] = -1;
+            int q = NV_FB_NSLOW - 1;
+            if (!x.sact.empty()) {
+                q = -1;
+                for (int z = 0; z < NV_FB_NSLOW - 1; ++z)
+                    if (k[0] == kSlow[z][0] && k[1] == kSlow[z][1] && k[2] == kSlow[z][2]) q = z;
+                if (q < 0) return;
+            }
+            sindex[x.skey] = {q, (int)byshape[q].size()};
+            byshape[q].push_back(S);
+            it = sindex.find(x.skey);
+        }
+        NvFbS &S = byshape[it->second.first][it->second.second];
+        S.cm[ch->second] = op.term_dev[ti];
+        S.chmask |= 1 << ch->second;
+        if (ch->second == 0 && t.diag) S.excl0 = 1;
+    }
+    for (int q = 0; q < NV_FB_NSLOW; ++q) {
+        op.fb_sgrp[q] = (int32_t)op.fb_S.size();
+        op.fb_S.insert(op.fb_S.end(), byshape[q].begin(), byshape[q].end());
+    }
+    op.fb_sgrp[NV_FB_NSLOW] = (int32_t)op.fb_S.size();
+    op.fb_nch = (int32_t)chkeys.size();
+    op.fb_mf = mf;
+    op.fb_P = (int32_t)P;
+    op.fb_halo = halo;
+    // out-of-basis slow steps read the first 64 rows of the margin
+    op.fb_pad = (int32_t)std::max<int64_t>(pad, 64);
+    op.fb = true;
 }
 
 bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
@@ -448,6 +627,8 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
         for (int q = 0; q <= NV_NSHAPES; ++q) op.shape_grp[q] = 0;
     }
     build_two_pass(op, code_of);
+    std::vector<std::array<int, 4>> fb_tabcodes;
+    build_fast_block(op, code_of, fb_tabcodes);
     // constant block matrix, gated by the diagonal-block list (Source/top.f90:528-553)
     op.dconst.assign(ns * ns, 0.0);
     int napplied = 0;
@@ -472,6 +653,12 @@ bool build_operator(const nv_problem &pr, Operator &op, std::string &err) {
             op.ftab[(size_t)kv.second * op.nmax + n] = hoelem(p, std::min(n, n2), std::abs(delta), op.quirks);
         }
     }
+    op.fb_tab.assign(fb_tabcodes.size() * op.nmax * 4, 0.0);
+    for (size_t k = 0; k < fb_tabcodes.size(); ++k)
+        for (int64_t n = 0; n < op.nmax; ++n)
+            for (int i = 0; i < 4; ++i)
+                if (fb_tabcodes[k][i] >= 0)
+                    op.fb_tab[(k * op.nmax + n) * 4 + i] = op.ftab[(size_t)fb_tabcodes[k][i] * op.nmax + n];
 
     // ---- start-vector data ----
     op.neworigin = pr.neworigin;

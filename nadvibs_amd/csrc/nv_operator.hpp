@@ -60,6 +60,16 @@ struct Operator {
     int32_t hv2_idcm[NV_HV2_NG] = {0};
     int32_t hv2_gmask = 0;
     NvSigma hv2_sig[NV_HV2_NG];
+    // fast-block H*v tables (see NvFbDev)
+    bool fb = false;
+    int32_t fb_mf = 0, fb_P = 0, fb_halo = 0, fb_pad = 0, fb_nch = 0;
+    std::vector<int32_t> fb_tabmode;       // [ntab] mode of each factor row
+    std::vector<double> fb_tab;            // [ntab][nmax][4]
+    std::vector<NvFbS> fb_S;               // grouped by slow shape
+    int32_t fb_sgrp[NV_FB_NSLOW + 1] = {0};
+    int32_t fb_chs[NV_FB_NCH + 1] = {0};
+    std::vector<NvStep> fb_steps;          // fast steps of the channels, then of the pure terms
+    std::vector<NvFbPure> fb_pure;
 
     // start vector data
     int32_t neworigin = 0;

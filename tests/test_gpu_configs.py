@@ -7,7 +7,8 @@
     converged roots of the mixed run within 1e-3 cm^-1 of FP64
     (Source/top.f90:672-842, :844-1005).
   * 88.5M-row operator (4 states, 10 modes, ordr 3): H*v at 4096 random rows
-    against the oracle's row restatement (FP64 and FP32), the symmetry
+    against the oracle's row restatement (FP64 and FP32; the fast-block form,
+    the default there, and the kernels without it), the symmetry
     x.(H y) = y.(H x) over the whole vector, and the first three iterations of
     both precisions against the oracle's recurrence computed exactly on the
     rows the Krylov vectors reach (the start vector lives on row 0, and each
@@ -98,16 +99,36 @@ def cfg3():
     return p, Oracle(p)
 
 
+def _form(form, monkeypatch):
+    """The default H*v kernel at this size (the fast-block form), or the
+    kernels that serve without it (NV_HV_FB=0: FP32 shaped, FP64 two-pass)."""
+    if form == "fast-block":
+        monkeypatch.delenv("NV_HV_FB", raising=False)
+    else:
+        monkeypatch.setenv("NV_HV_FB", "0")
+
+
+def _check_form(eng, form, prec):
+    info = eng.hv_kernel(prec)
+    if form == "fast-block":
+        assert info.startswith("generic fast-block"), info
+    else:
+        assert info.startswith("generic two-pass" if prec == "f64" else "generic shaped"), info
+    return info
+
+
+@pytest.mark.parametrize("form", ["fast-block", "no-fast-block"])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 2e-6)])
-def test_cfg3_hv_rows_match_oracle(cfg3, prec, tol):
+def test_cfg3_hv_rows_match_oracle(cfg3, prec, tol, form, monkeypatch):
     p, orc = cfg3
+    _form(form, monkeypatch)
     rng = np.random.default_rng(5)
     x = rng.standard_normal(p.n)
     rows = np.sort(rng.choice(orc.dimen, 4096, replace=False))
     rows[:4] = [0, 1, orc.dimen - 2, orc.dimen - 1]  # the corners of the basis
     ref = orc.hv_rows(x, rows)
     with Engine(p) as eng:
-        _say(f"cfg3 H*v {prec}: kernel {eng.hv_kernel(prec)}")
+        _say(f"cfg3 H*v {prec}: kernel {_check_form(eng, form, prec)}")
         y = eng.hv(x, prec)
     got = y.reshape(p.nstates, -1)[:, rows].T
     err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
@@ -147,10 +168,12 @@ def _rows_within(nfunc, qmax):
     return np.array(sorted(out), dtype=np.int64)
 
 
-def test_cfg3_first_iterations_match_exact_recurrence(cfg3):
+@pytest.mark.parametrize("form", ["fast-block", "no-fast-block"])
+def test_cfg3_first_iterations_match_exact_recurrence(cfg3, form, monkeypatch):
     """alpha(1:3), beta(0:2): the oracle's recurrence evaluated on the rows the
     Krylov vectors reach (q_k lives within 3(k-1) quanta of row 0)."""
     p, orc = cfg3
+    _form(form, monkeypatch)
     ns, D = p.nstates, orc.dimen
     q = orc.start_vector()
     q_prev = np.zeros_like(q)
@@ -171,7 +194,7 @@ def test_cfg3_first_iterations_match_exact_recurrence(cfg3):
     with Engine(p) as eng:
         for prec, tol in (("f64", 1e-11), ("f32", 5e-6)):
             eng.lanczos_init(niter=3, reorthog=0, prec=prec)
-            _say(f"cfg3 first 3 iterations, {prec}, kernel {eng.hv_kernel(prec)}")
+            _say(f"cfg3 first 3 iterations, {prec}, kernel {_check_form(eng, form, prec)}")
             eng.run(1, 3)
             a, b, _ = eng.coefficients()
             da = np.max(np.abs(a - alpha) / np.abs(alpha))

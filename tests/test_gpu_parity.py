@@ -196,17 +196,24 @@ def test_lanczos_bf16_input_tracks_fp64():
     assert rel.max() > 1e-7  # it is not running the FP32 path
 
 
-@pytest.mark.parametrize("form", ["shaped", "steps"])
-def test_generic_two_row_kernel_on_a_large_operator(form):
-    """4,194,304 rows in FP32, the generic H*v in both of its forms (the
-    step-table form runs two rows per thread from this size on): its first
-    coefficients agree with the FP64 oracle to the FP32 level, as on cfg1."""
+@pytest.mark.parametrize("form", ["shaped", "steps", "fast-block"])
+def test_generic_two_row_kernel_on_a_large_operator(form, monkeypatch):
+    """4,194,304 rows in FP32, the generic H*v in its forms (the step-table
+    form runs two rows per thread from this size on; the fast-block form is
+    the default at this size and NV_HV_FB=0 leaves the one-pass shaped one):
+    its first coefficients agree with the FP64 oracle to the FP32 level, as on
+    cfg1."""
+    if form == "fast-block":
+        monkeypatch.delenv("NV_HV_FB", raising=False)
+    else:
+        monkeypatch.setenv("NV_HV_FB", "0")
     p = make_problem(2, [16, 16, 16, 16, 16, 4], 2, (400, 3000), 250.0, 60.0, 40.0, seed=13)
     assert p.n // p.nstates == 1 << 22
     ref = Oracle(p).lanczos(3)
     with Engine(p) as eng:
-        eng.select_hv_kernel(True if form == "shaped" else "steps")
-        assert eng.hv_kernel("f32").startswith("generic " + ("shaped" if form == "shaped" else "step-table"))
+        eng.select_hv_kernel("steps" if form == "steps" else True)
+        kind = {"shaped": "shaped", "steps": "step-table", "fast-block": "fast-block"}[form]
+        assert eng.hv_kernel("f32").startswith("generic " + kind), eng.hv_kernel("f32")
         eng.lanczos_init(niter=3, prec="f32")
         eng.run(1, 3)
         a, b, _ = eng.coefficients()
@@ -458,6 +465,54 @@ def test_two_pass_hv_matches_oracle(case, variant, monkeypatch):
             assert info.startswith("generic two-pass"), info
             if prec == "f32":
                 assert ("MFMA" in info) == (variant == "mfma-skip"), info
+            err = np.max(np.abs(eng.hv(x, prec) - ref)) / np.max(np.abs(ref))
+            assert err <= tol_hv, (prec, err)
+            eng.lanczos_init(niter=12, reorthog=0, prec=prec)
+            eng.run(1, 12)
+            a, b, _ = eng.coefficients()
+            assert np.max(np.abs(a - r["alpha"]) / np.abs(r["alpha"])) <= tol_ab, prec
+            assert np.max(np.abs(b - r["beta"]) / np.abs(r["beta"])) <= tol_ab, prec
+
+
+FAST_BLOCK = [
+    # ns, nfunc, ordr, cubic, offdiag_even, quirks: operators whose trailing modes span 33..64
+    # rows (the fast-block form, forced on at any size): 64, 36, 40 and 49 rows per block, slow
+    # modes of one function, up to ten states, ordr 1-3, both quirk sets
+    (4, [3, 4, 4, 4], 3, 10.0, 0.0, 3),
+    (3, [3, 2, 6, 6], 3, 10.0, 30.0, 3),
+    (2, [4, 3, 5, 8], 2, 0.0, 0.0, 3),
+    (1, [5, 7, 7], 3, 10.0, 0.0, 0),
+    (3, [3, 1, 4, 4, 4], 3, 10.0, 5.0, 3),
+    (6, [3, 4, 4, 4], 2, 0.0, 10.0, 3),
+    (2, [3, 8, 8], 1, 0.0, 0.0, 3),
+    (4, [5, 2, 2, 6, 6], 3, 10.0, 10.0, 3),
+    (3, [4, 1, 4, 4, 4], 3, 10.0, 5.0, 0),
+    (10, [2, 5, 8], 2, 0.0, 5.0, 3),
+    (4, [5, 4, 4, 4], 3, 10.0, 0.0, 3),
+]
+
+
+@pytest.mark.parametrize("case", FAST_BLOCK, ids=[f"b{i}" for i in range(len(FAST_BLOCK))])
+def test_fast_block_hv_matches_oracle(case, monkeypatch):
+    """The fast-block H*v (one wave per block of the trailing modes' rows;
+    slow steps wave-uniform, fast steps through LDS) forced on small
+    operators: standalone H*v in both precisions and the first Lanczos
+    coefficients against the oracle."""
+    monkeypatch.setenv("NV_HV_FB", "1")
+    monkeypatch.delenv("NV_HV_KERNEL", raising=False)
+    ns, nf, ordr, cub, offe, quirks = case
+    p = make_problem(ns, nf, ordr, (400, 3000), 250.0, 60.0, 40.0, cubic=cub, seed=5, quirks=quirks,
+                     offdiag_even=offe)
+    print(f"case {case} n={p.n}", flush=True)
+    orc = Oracle(p)
+    x = np.random.default_rng(3).standard_normal(p.n)
+    ref = orc.hv(x)
+    r = orc.lanczos(12)
+    with Engine(p) as eng:
+        for prec, tol_hv, tol_ab in (("f64", 1e-13, 1e-11), ("f32", 2e-6, 5e-6)):
+            info = eng.hv_kernel(prec)
+            print(f"{prec}: {info}", flush=True)
+            assert info.startswith("generic fast-block"), info
             err = np.max(np.abs(eng.hv(x, prec) - ref)) / np.max(np.abs(ref))
             assert err <= tol_hv, (prec, err)
             eng.lanczos_init(niter=12, reorthog=0, prec=prec)
