@@ -1337,13 +1337,14 @@ template <typename TAcc> __device__ __forceinline__ FbWave<TAcc> fb_prologue(con
 // Iteration H*v in the fast-block form: y = offdiag(H) q_i, alpha partials.
 // Waves per SIMD the compiler must fit (register budget): left free, the
 // FP32 kernel holds 87 VGPRs (5 waves) and the FP64 one 132 (3 waves); at 8
-// and 5 waves they run 13.2 and 30.3 ms against 15.1 and 36.2 ms (four
-// states, 22M rows; 7 and 4 waves: 13.8 and 30.5 ms).
+// and 4 waves they run 13.2 and 30.5 ms against 15.1 and 36.2 ms (four
+// states, 22M rows; FP32 at 7 waves: 13.8 ms; FP64 at 5: 30.3 ms, but it
+// spills 12 GB of scratch per launch).
 #ifndef NV_FB_WAVES32
 #define NV_FB_WAVES32 8
 #endif
 #ifndef NV_FB_WAVES64
-#define NV_FB_WAVES64 5
+#define NV_FB_WAVES64 4
 #endif
 // (Above four states the ten channels of NS values need more registers: the
 // budget follows the state count.)
@@ -1453,23 +1454,46 @@ __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzStat
     if (tid == 0) st.partials[blockIdx.x] = bs;
 }
 
+
+// One interleaved row of K values (the H*v input copies), aligned for one
+// wide store where its size allows.
+__host__ __device__ constexpr size_t il_align(size_t b) { return b % 16 == 0 ? 16 : b % 8 == 0 ? 8 : 4; }
+template <typename T, int K> struct alignas(il_align(sizeof(T) * K) < alignof(T) ? alignof(T)
+                                                                                  : il_align(sizeof(T) * K)) IlRow {
+    T v[K];
+};
+
 // update_lanczos (Source/top.f90:646-670): q_i = r / beta(i-1) in place, and
-// the copy of q_i in the H*v input precision.
-template <typename TIn>
-__global__ __launch_bounds__(kBlock) void normalize_kernel(int64_t N, int64_t D, int nsp, LzState st) {
+// the copies of q_i that H*v and re-orthogonalisation read.  A thread takes
+// the NS states of row r, so an interleaved copy gets one row store.
+template <int NS, typename TIn>
+__global__ __launch_bounds__(kBlock) void normalize_rows_kernel(int64_t N, int64_t D, LzState st) {
+    constexpr int NSP = nv_lo_stride(NS);
     const int64_t it = *st.iter;
     double *q = st.V + slot_of(it, st.nslots) * N;
     const double sc = 1.0 / st.beta[it - 1];
     TIn *lo = reinterpret_cast<TIn *>(st.vlo);
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (int64_t)gridDim.x * kBlock) {
-        const double v = q[i] * sc;
-        q[i] = v;
-        if (lo || st.v64i) {
-            const int64_t s = i / D;
-            if (lo) lo[(i - s * D) * nsp + s] = from_d<TIn>(v);
-            if (st.v64i) st.v64i[(i - s * D) * nsp + s] = v;
+    float *vf = st.Vf ? st.Vf + slot_of(it, st.nslots) * st.ldf : nullptr;
+    for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < D; r += (int64_t)gridDim.x * kBlock) {
+        double v[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            v[s] = q[s * D + r] * sc;
+            q[s * D + r] = v[s];
+            if (vf) vf[s * D + r] = (float)v[s];
         }
-        if (st.Vf) st.Vf[slot_of(it, st.nslots) * st.ldf + i] = (float)v;
+        if (lo) {
+            IlRow<TIn, NSP> row;
+#pragma unroll
+            for (int k = 0; k < NSP; ++k) row.v[k] = from_d<TIn>(k < NS ? v[k] : 0.0);
+            reinterpret_cast<IlRow<TIn, NSP> *>(lo)[r] = row;
+        }
+        if (st.v64i) {
+            IlRow<double, NSP> row;
+#pragma unroll
+            for (int k = 0; k < NSP; ++k) row.v[k] = k < NS ? v[k] : 0.0;
+            reinterpret_cast<IlRow<double, NSP> *>(st.v64i)[r] = row;
+        }
     }
 }
 
@@ -1499,6 +1523,8 @@ __global__ __launch_bounds__(kBlock) void norm_kernel(const double *v, int64_t n
 __global__ __launch_bounds__(kBlock) void finish_sum_kernel(const double *partials, int nparts, double *out,
                                                            const int64_t *iter, int take_sqrt) {
     double s = 0.0;
+    // loads unrolled ahead of the adds, which keep their order
+#pragma unroll 8
     for (int i = threadIdx.x; i < nparts; i += kBlock) s += partials[i];
     s = block_sum(s);
     if (threadIdx.x == 0) {
@@ -1867,17 +1893,23 @@ hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, h
     return launch_finish_sum(st.partials, hv_blocks(op), st.beta, st.iter, 1, s);
 }
 
-hipError_t launch_normalize(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
-    const int g = grid_stride_blocks(op.N);
-    const int64_t N = op.N, D = op.D;
-    const int nsp = nv_lo_stride(op.ns);
+template <int NS>
+static hipError_t normalize_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    const int g = grid_stride_blocks(op.D);
     if (prec == 0 || st.vlo == nullptr)
-        hipLaunchKernelGGL((normalize_kernel<double>), dim3(g), dim3(kBlock), 0, s, N, D, nsp, st);
+        hipLaunchKernelGGL((normalize_rows_kernel<NS, double>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N,
+                           (int64_t)op.D, st);
     else if (prec == 1)
-        hipLaunchKernelGGL((normalize_kernel<float>), dim3(g), dim3(kBlock), 0, s, N, D, nsp, st);
+        hipLaunchKernelGGL((normalize_rows_kernel<NS, float>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N,
+                           (int64_t)op.D, st);
     else
-        hipLaunchKernelGGL((normalize_kernel<__hip_bfloat16>), dim3(g), dim3(kBlock), 0, s, N, D, nsp, st);
+        hipLaunchKernelGGL((normalize_rows_kernel<NS, __hip_bfloat16>), dim3(g), dim3(kBlock), 0, s,
+                           (int64_t)op.N, (int64_t)op.D, st);
     return hipGetLastError();
+}
+
+hipError_t launch_normalize(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+    NV_DISPATCH_NS(normalize_ns, op, prec, st, s)
 }
 
 // y[r * nsp + s] = (float)x[s * D + r] (the interleaved FP32 copy).

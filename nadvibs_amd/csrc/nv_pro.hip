@@ -488,6 +488,7 @@ __global__ __launch_bounds__(kBlock) void pro_axpy_kernel(int64_t N, ProDev p, L
 __global__ __launch_bounds__(kBlock) void pro_beta_kernel(ProDev p, LzState st, int nparts) {
     if (*p.njl == 0) return;
     double s = 0.0;
+#pragma unroll 8
     for (int i = threadIdx.x; i < nparts; i += kBlock) s += st.partials[i];
     s = pro_block_sum(s);
     if (threadIdx.x == 0) st.beta[*st.iter] = sqrt(s);
@@ -502,8 +503,10 @@ __global__ __launch_bounds__(kDecide) void pro_step_kernel(ProDev p, LzState st,
     __shared__ double sh_beta;
     const int64_t it = *st.iter;
     double v = 0.0;
-    if (threadIdx.x < kBlock)
+    if (threadIdx.x < kBlock) {
+#pragma unroll 8
         for (int i = threadIdx.x; i < nparts; i += kBlock) v += st.partials[i];
+    }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (threadIdx.x < kBlock && (threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
