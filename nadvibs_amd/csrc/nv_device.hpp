@@ -124,7 +124,8 @@ struct NvFbS {
     int32_t dtab[2];       // their factor rows; entry 0 is h(p, n, 0)
     int32_t cm[NV_FB_NCH]; // per channel: coupling matrix (row of cmat) of the term (S, F), or -1
     int32_t excl0;         // channel 0 leaves out the middle step Delta_s = 0 (all stepping slots even)
-    int32_t chmask;        // bit c: cm[c] >= 0
+    int32_t nchk;          // channels to visit: 1 + the last c with cm[c] >= 0 (channels are
+                           // ordered by the degree of their fast monomial)
 };
 struct NvFbPure {
     int32_t cm;            // coupling matrix

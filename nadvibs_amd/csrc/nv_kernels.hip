@@ -1091,6 +1091,26 @@ template <> __device__ __forceinline__ const float *fbtab_of<float>(const NvOpDe
 // The four factors h(p, n, -s + 2i) of a slow slot at the wave's digit.
 template <typename T> struct Fb4 { T v[4]; };
 
+// V_c += C_{cm[c]} w for the channels c < NC that the slow monomial d feeds
+// (channel 0 without the middle step when d.excl0).
+template <int NS, typename TAcc, int NC, bool ODD>
+__device__ __forceinline__ void fb_mix(const TAcc *__restrict__ cm, const NvFbS &d, const TAcc (&w)[NS],
+                                       const TAcc (&wf)[NS], TAcc (&V)[NV_FB_NCH][NS]) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        if (d.cm[c] < 0) continue;
+        const TAcc *C = cm + (size_t)d.cm[c] * NS * NS;
+        const bool ex = c == 0 && ODD && d.excl0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            TAcc a = V[c][s];
+#pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2) a += C[s * NS + s2] * (ex ? w[s2] : wf[s2]);
+            V[c][s] = a;
+        }
+    }
+}
+
 // The slow monomials [t0, t1) of shape (K0, K1, K2), gathered and mixed into
 // the channels V; vl is the lane's byte offset in its block, rb the byte
 // offset of the block's first row in the padded buffer, fw/fwm the wave's
@@ -1122,6 +1142,8 @@ __device__ __forceinline__ void fb_slow(const NvOpDev &op, const XRes<NS, TIn> &
             m2 = __builtin_amdgcn_readfirstlane(fwm[d.tab[2]]);
         }
         const unsigned b0 = rb + (unsigned)d.base * kEl;
+        const unsigned ib0 = (unsigned)d.inc[0] * kEl, ib1 = (unsigned)d.inc[1] * kEl,
+                       ib2 = (unsigned)d.inc[2] * kEl;
         // w: every step but the middle one of an all-odd shape, wc: that one
         TAcc w[NS], wc[NS];
 #pragma unroll
@@ -1136,15 +1158,15 @@ __device__ __forceinline__ void fb_slow(const NvOpDev &op, const XRes<NS, TIn> &
                     unsigned so = rb;
                     if constexpr (K0 > 0) {
                         h = f0.v[i0];
-                        so = b0 + (unsigned)(i0 * d.inc[0]) * kEl;
+                        so = b0 + (unsigned)i0 * ib0;
                     }
                     if constexpr (K1 > 0) {
                         h *= f1.v[i1];
-                        so += (unsigned)(i1 * d.inc[1]) * kEl;
+                        so += (unsigned)i1 * ib1;
                     }
                     if constexpr (K2 > 0) {
                         h *= f2.v[i2];
-                        so += (unsigned)(i2 * d.inc[2]) * kEl;
+                        so += (unsigned)i2 * ib2;
                     }
                     const bool in = ((m0 >> i0) & (K1 > 0 ? m1 >> i1 : 1) & (K2 > 0 ? m2 >> i2 : 1) & 1) != 0;
                     const bool mid =
@@ -1164,18 +1186,7 @@ __device__ __forceinline__ void fb_slow(const NvOpDev &op, const XRes<NS, TIn> &
         TAcc wf[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) wf[s] = w[s] + wc[s];
-        for (int c = 0; c < NV_FB_NCH; ++c) {
-            if (d.cm[c] < 0) continue;
-            const TAcc *C = cm + (size_t)d.cm[c] * NS * NS;
-            const bool ex = c == 0 && odd && d.excl0;
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                TAcc a = V[c][s];
-#pragma unroll
-                for (int s2 = 0; s2 < NS; ++s2) a += C[s * NS + s2] * (ex ? w[s2] : wf[s2]);
-                V[c][s] = a;
-            }
-        }
+        fb_mix<NS, TAcc, NV_FB_NCH, odd>(cm, d, w, wf, V);
     }
 }
 

@@ -292,6 +292,28 @@ static void build_fast_block(Operator &op, CodeOf &code_of, std::vector<std::arr
             chterm.push_back((int)ti);
         }
     }
+    // channels in order of the degree of their fast monomial, so that a slow monomial of high
+    // degree (which pairs with low-degree fast monomials only) visits a short prefix
+    {
+        std::vector<int> ord(chkeys.size());
+        for (size_t c = 0; c < ord.size(); ++c) ord[c] = (int)c;
+        auto degree = [&](const Key &k) {
+            int d = 0;
+            for (auto &mp : k) d += mp.second;
+            return d;
+        };
+        std::stable_sort(ord.begin() + 1, ord.end(),
+                         [&](int a, int b) { return degree(chkeys[a]) < degree(chkeys[b]); });
+        std::vector<Key> k2;
+        std::vector<int> t2;
+        for (int c : ord) {
+            k2.push_back(chkeys[c]);
+            t2.push_back(chterm[c]);
+        }
+        chkeys.swap(k2);
+        chterm.swap(t2);
+        for (size_t c = 0; c < chkeys.size(); ++c) chan[chkeys[c]] = (int)c;
+    }
     // the fast steps of a term's fast slots (all of them; the Delta_f = 0 one left out when
     // skip0), appended to fb_steps
     int halo = 0;
@@ -381,7 +403,7 @@ static void build_fast_block(Operator &op, CodeOf &code_of, std::vector<std::arr
         }
         NvFbS &S = byshape[it->second.first][it->second.second];
         S.cm[ch->second] = op.term_dev[ti];
-        S.chmask |= 1 << ch->second;
+        S.nchk = std::max<int32_t>(S.nchk, ch->second + 1);
         if (ch->second == 0 && t.diag) S.excl0 = 1;
     }
     for (int q = 0; q < NV_FB_NSLOW; ++q) {
