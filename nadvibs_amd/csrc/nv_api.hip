@@ -45,7 +45,7 @@ struct nv_ctx {
 
     // graphs of one iteration: [0] plain, [1] with re-orthogonalisation,
     // [2] re-orthogonalisation with the exact overlap check
-    hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t graphs[4] = {nullptr, nullptr, nullptr, nullptr};  // kinds 0-2, and kIterBatch of kind 0
     hipGraphExec_t graph = nullptr;
 
     // operator-specialised H*v kernels per precision (F64, F32)
@@ -895,8 +895,11 @@ int32_t nv_partial_orthog(nv_ctx *c, int64_t it) {
 // The kernels of one iteration, in the reference's order; the iteration
 // number is advanced on the device so that a captured sequence replays for
 // every iteration.
-static hipError_t enqueue_iteration(nv_ctx *c, int pro, int check) {
-    hipError_t e = nv::launch_set_iter(c->d_iter, 0, 1, c->stream);
+// One iteration on the stream.  set: advance the device counter first (a
+// separate launch); advance: the beta finish advances it for the next
+// iteration instead (runs of several iterations in one graph).
+static hipError_t enqueue_iteration(nv_ctx *c, int pro, int check, bool set = true, bool advance = false) {
+    hipError_t e = set ? nv::launch_set_iter(c->d_iter, 0, 1, c->stream) : hipSuccess;
     if (e == hipSuccess) e = nv::launch_normalize(c->dev, c->opts.prec, c->st, c->stream);
     if (e == hipSuccess) e = launch_hv_iter_any(c);
     if (pro && !check && nv::pro_fusable(c->pro)) {
@@ -905,13 +908,19 @@ static hipError_t enqueue_iteration(nv_ctx *c, int pro, int check) {
         if (e == hipSuccess) e = nv::pro_step_fused(c->pro, c->dev, c->st, nv::hv_blocks(c->dev), c->stream);
         return e;
     }
-    if (e == hipSuccess) e = nv::launch_lanczos_step(c->dev, c->opts.prec, c->st, c->stream);
+    if (e == hipSuccess)
+        e = nv::launch_lanczos_step(c->dev, c->opts.prec, c->st, c->stream, advance ? c->d_iter : nullptr);
     if (pro) {
         if (e == hipSuccess) e = nv::pro_set_omega(c->pro, c->dev, c->st, check, c->stream);
         if (e == hipSuccess) e = nv::pro_partial_orthog(c->pro, c->dev, c->st, c->stream);
     }
     return e;
 }
+
+// Iterations per graph launch in runs without re-orthogonalisation (graph kind
+// 3): the host launch and the gaps around it were about a third of a 68 us
+// iteration on a 1.38M-element operator with one iteration per graph.
+constexpr int64_t kIterBatch = 16;
 
 static hipError_t iteration_graph(nv_ctx *c, int kind, hipGraphExec_t *out) {
     if (c->graphs[kind]) {
@@ -921,7 +930,12 @@ static hipError_t iteration_graph(nv_ctx *c, int kind, hipGraphExec_t *out) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    hipError_t e2 = enqueue_iteration(c, kind >= 1, kind == 2);
+    hipError_t e2 = hipSuccess;
+    if (kind == 3)
+        for (int64_t k = 0; k < kIterBatch && e2 == hipSuccess; ++k)
+            e2 = enqueue_iteration(c, 0, 0, k == 0, k + 1 < kIterBatch);
+    else
+        e2 = enqueue_iteration(c, kind >= 1, kind == 2);
     e = hipStreamEndCapture(c->stream, &g);
     if (e2 != hipSuccess) e = e2;
     if (e == hipSuccess) e = hipGraphInstantiate(&c->graphs[kind], g, nullptr, nullptr, 0);
@@ -950,6 +964,16 @@ int32_t nv_lanczos_run(nv_ctx *c, int64_t iiter, int64_t niter) {
                 if (!rc) rc = nv_partial_orthog(c, i);
             }
             if (rc) return rc;
+            continue;
+        }
+        if (!orthog && !c->cp && i + kIterBatch - 1 <= niter) {
+            // kIterBatch iterations in one launch (the device counter advances in each)
+            hipGraphExec_t gb = nullptr;
+            NV_HIP(c, iteration_graph(c, 3, &gb));
+            NV_HIP(c, hipGraphLaunch(gb, c->stream));
+            i += kIterBatch - 1;
+            c->next_update = i + 1;
+            c->last_iter = i;
             continue;
         }
         int kind = 0;

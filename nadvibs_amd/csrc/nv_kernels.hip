@@ -1530,9 +1530,10 @@ __global__ __launch_bounds__(kBlock) void norm_kernel(const double *v, int64_t n
 
 // Sums nparts per-block partials in block order (one block; the kernel
 // boundary makes the partials visible) and stores the sum, or its square
-// root, at out[*iter] (iter null: out[0]).
+// root, at out[*iter] (iter null: out[0]); then, with advance, sets
+// *advance = *iter + 1 (the next iteration's counter, in place of a launch).
 __global__ __launch_bounds__(kBlock) void finish_sum_kernel(const double *partials, int nparts, double *out,
-                                                           const int64_t *iter, int take_sqrt) {
+                                                           const int64_t *iter, int take_sqrt, int64_t *advance) {
     double s = 0.0;
     // loads unrolled ahead of the adds, which keep their order
 #pragma unroll 8
@@ -1541,12 +1542,14 @@ __global__ __launch_bounds__(kBlock) void finish_sum_kernel(const double *partia
     if (threadIdx.x == 0) {
         const int64_t k = iter ? *iter : 0;
         out[k] = take_sqrt ? sqrt(s) : s;
+        if (advance) *advance = k + 1;
     }
 }
 
 hipError_t launch_finish_sum(const double *partials, int nparts, double *out, const int64_t *iter, int take_sqrt,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kBlock), 0, s, partials, nparts, out, iter, take_sqrt);
+                             hipStream_t s, int64_t *advance) {
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kBlock), 0, s, partials, nparts, out, iter, take_sqrt,
+                       advance);
     return hipGetLastError();
 }
 
@@ -1898,10 +1901,10 @@ hipError_t launch_lanczos_step_only(const NvOpDev &op, int prec, const LzState &
     return step_any(op, prec, st, s);
 }
 
-hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
+hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s, int64_t *advance) {
     hipError_t e = step_any(op, prec, st, s);
     if (e != hipSuccess) return e;
-    return launch_finish_sum(st.partials, hv_blocks(op), st.beta, st.iter, 1, s);
+    return launch_finish_sum(st.partials, hv_blocks(op), st.beta, st.iter, 1, s, advance);
 }
 
 template <int NS>

@@ -84,14 +84,16 @@ hipError_t launch_hv_iter(const NvOpDev &op, int prec, const LzState &st, hipStr
 // The H*v kernel of launch_hv_iter alone (its alpha partials are not summed).
 hipError_t launch_hv_iter_only(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
 // lanczos_step: r_i = H q_i - alpha q_i - beta(i-1) q_{i-1}, beta(i) = |r_i|.
-hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
+hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, hipStream_t s,
+                               int64_t *advance = nullptr);
 // Its kernel alone: the residual and the block partials of |r|^2 (hv_blocks(op) of them).
 hipError_t launch_lanczos_step_only(const NvOpDev &op, int prec, const LzState &st, hipStream_t s);
 // Writes the start vector: zero except vib index 0 of each state, from vals[ns].
 hipError_t launch_fill_start(double *v, int64_t D, int ns, const double *vals_dev, hipStream_t s);
-// out[*iter] (or out[0]) = sum (or sqrt of the sum) of nparts partials.
+// out[*iter] (or out[0]) = sum (or sqrt of the sum) of nparts partials; with
+// advance, then *advance = *iter + 1.
 hipError_t launch_finish_sum(const double *partials, int nparts, double *out, const int64_t *iter, int take_sqrt,
-                             hipStream_t s);
+                             hipStream_t s, int64_t *advance = nullptr);
 // beta[0] = |v| over n doubles.
 hipError_t launch_norm_to(const double *v, int64_t n, double *out, double *partials, unsigned *counter,
                           hipStream_t s);

@@ -317,6 +317,30 @@ def test_captured_pro_iterations_equal_the_reference_loop(prec):
     assert np.array_equal(a1, a2) and np.array_equal(b1, b2) and np.array_equal(w1, w2)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batched_graph_iterations_equal_the_reference_loop(prec):
+    """Without re-orthogonalisation run() replays graphs of 16 iterations, in
+    which the beta finish advances the iteration counter in place of a
+    launch; runs of 1..40 iterations split into such graphs, single captured
+    iterations and the per-call path give the same bits as the reference's
+    loop (update_lanczos, Hv, lanczos_step)."""
+    p = workload("cfg1")
+    n = 75
+    with Engine(p) as eng:
+        eng.lanczos_init(niter=n, reorthog=0, prec=prec)
+        for i in range(1, n + 1):
+            eng.update_lanczos(i)
+            eng.Hv(i)
+            eng.lanczos_step(i)
+        a0, b0, _ = eng.coefficients()
+        for cuts in ([1, n], [1, 17, 18, 40, 41, n], [1, 5, 6, 38, 39, n]):
+            eng.lanczos_init(niter=n, reorthog=0, prec=prec)
+            for lo, hi in zip(cuts[::2], cuts[1::2]):
+                eng.run(lo, hi)
+            a, b, _ = eng.coefficients()
+            assert np.array_equal(a, a0) and np.array_equal(b, b0), cuts
+
+
 def test_pro_with_noise_keeps_orthogonality():
     p = workload("cfg1")
     with Engine(p) as eng:
