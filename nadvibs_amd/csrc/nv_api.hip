@@ -264,7 +264,8 @@ int32_t nv_create(const nv_problem *prob, int32_t device, nv_ctx **out) {
             f.halo = op.fb_halo;
             f.pad = op.fb_pad;
             f.nch = op.fb_nch;
-            for (int q = 0; q <= NV_FB_NSLOW; ++q) f.sgrp[q] = op.fb_sgrp[q];
+            for (int ph = 0; ph < 3; ++ph)
+                for (int q = 0; q <= NV_FB_NSLOW; ++q) f.sgrp[ph][q] = op.fb_sgrp[ph][q];
             for (int q = 0; q <= NV_FB_NCH; ++q) f.chs[q] = op.fb_chs[q];
             f.npure = (int32_t)op.fb_pure.size();
             f.ntab = (int32_t)op.fb_tabmode.size();

@@ -124,8 +124,8 @@ struct NvFbS {
     int32_t dtab[2];       // their factor rows; entry 0 is h(p, n, 0)
     int32_t cm[NV_FB_NCH]; // per channel: coupling matrix (row of cmat) of the term (S, F), or -1
     int32_t excl0;         // channel 0 leaves out the middle step Delta_s = 0 (all stepping slots even)
-    int32_t nchk;          // channels to visit: 1 + the last c with cm[c] >= 0 (channels are
-                           // ordered by the degree of their fast monomial)
+    int32_t nchk;          // 1 + the last channel c with cm[c] >= 0 (channels are ordered by the
+                           // degree of their fast monomial)
 };
 struct NvFbPure {
     int32_t cm;            // coupling matrix
@@ -148,8 +148,8 @@ struct NvFbDev {
     const int32_t *tabmode;  // [ntab] mode of each factor row
     const double *tab64;   // [ntab][nmax][4]
     const float *tab32;
-    const NvFbS *S;        // grouped by slow shape
-    int32_t sgrp[NV_FB_NSLOW + 1];
+    const NvFbS *S;        // grouped by phase, then by slow shape
+    int32_t sgrp[3][NV_FB_NSLOW + 1];  // phase 0: nchk > 4, phase 1: 1 < nchk <= 4, phase 2: nchk == 1
     int32_t nch;           // channels in use
     int32_t chs[NV_FB_NCH + 1];  // fast steps of channel c: [chs[c], chs[c+1]) (c >= 1)
     const NvStep *steps;   // fast steps: stride = lane offset; factors (mode, ftab row)

@@ -1115,7 +1115,7 @@ __device__ __forceinline__ void fb_mix(const TAcc *__restrict__ cm, const NvFbS 
 // the channels V; vl is the lane's byte offset in its block, rb the byte
 // offset of the block's first row in the padded buffer, fw/fwm the wave's
 // factor rows and their masks of slots inside the basis.
-template <int NS, typename TIn, typename TAcc, int K0, int K1, int K2>
+template <int NS, typename TIn, typename TAcc, int NC, int K0, int K1, int K2>
 __device__ __forceinline__ void fb_slow(const NvOpDev &op, const XRes<NS, TIn> &xs, unsigned vl, unsigned rb,
                                         const TAcc *fw, const int *fwm, int t0, int t1,
                                         TAcc (&V)[NV_FB_NCH][NS]) {
@@ -1186,7 +1186,7 @@ __device__ __forceinline__ void fb_slow(const NvOpDev &op, const XRes<NS, TIn> &
         TAcc wf[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) wf[s] = w[s] + wc[s];
-        fb_mix<NS, TAcc, NV_FB_NCH, odd>(cm, d, w, wf, V);
+        fb_mix<NS, TAcc, NC, odd>(cm, d, w, wf, V);
     }
 }
 
@@ -1222,20 +1222,35 @@ __device__ __forceinline__ void fb_offdiag(const NvOpDev &op, const XRes<NS, TIn
     for (int c = 0; c < NV_FB_NCH; ++c)
 #pragma unroll
         for (int s = 0; s < NS; ++s) V[c][s] = (TAcc)0;
-    const int *g = op.fb.sgrp;
-    fb_slow<NS, TIn, TAcc, 2, 0, 0>(op, xs, vl, rb, fw, fwm, g[0], g[1], V);
-    fb_slow<NS, TIn, TAcc, 3, 0, 0>(op, xs, vl, rb, fw, fwm, g[1], g[2], V);
-    fb_slow<NS, TIn, TAcc, 4, 0, 0>(op, xs, vl, rb, fw, fwm, g[2], g[3], V);
-    fb_slow<NS, TIn, TAcc, 2, 2, 0>(op, xs, vl, rb, fw, fwm, g[3], g[4], V);
-    fb_slow<NS, TIn, TAcc, 3, 2, 0>(op, xs, vl, rb, fw, fwm, g[4], g[5], V);
-    fb_slow<NS, TIn, TAcc, 2, 3, 0>(op, xs, vl, rb, fw, fwm, g[5], g[6], V);
-    fb_slow<NS, TIn, TAcc, 2, 2, 2>(op, xs, vl, rb, fw, fwm, g[6], g[7], V);
-    fb_slow<NS, TIn, TAcc, 0, 0, 0>(op, xs, vl, rb, fw, fwm, g[7], g[8], V);
+    // phase 0: the slow monomials that feed a channel beyond the first four (degree <= 1);
+    // those channels are complete afterwards and go through LDS, which frees their registers
+    const int *g0 = op.fb.sgrp[0];
+    fb_slow<NS, TIn, TAcc, NV_FB_NCH, 2, 0, 0>(op, xs, vl, rb, fw, fwm, g0[0], g0[1], V);
+    fb_slow<NS, TIn, TAcc, NV_FB_NCH, 0, 0, 0>(op, xs, vl, rb, fw, fwm, g0[7], g0[8], V);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) y[s] = V[0][s];
-#pragma unroll
-    for (int c = 1; c < NV_FB_NCH; ++c)
+    for (int c = 4; c < NV_FB_NCH; ++c)
         if (c < op.fb.nch) fb_fast<NS, TAcc>(op, op.fb.chs[c], op.fb.chs[c + 1], vb, lane, dig, ftab, tid, V[c], y);
+    // phase 1: channels 0..3 (slow degree <= 2)
+    const int *g1 = op.fb.sgrp[1];
+    fb_slow<NS, TIn, TAcc, 4, 2, 0, 0>(op, xs, vl, rb, fw, fwm, g1[0], g1[1], V);
+    fb_slow<NS, TIn, TAcc, 4, 3, 0, 0>(op, xs, vl, rb, fw, fwm, g1[1], g1[2], V);
+    fb_slow<NS, TIn, TAcc, 4, 2, 2, 0>(op, xs, vl, rb, fw, fwm, g1[3], g1[4], V);
+    fb_slow<NS, TIn, TAcc, 4, 0, 0, 0>(op, xs, vl, rb, fw, fwm, g1[7], g1[8], V);
+#pragma unroll
+    for (int c = 1; c < 4; ++c)
+        if (c < op.fb.nch) fb_fast<NS, TAcc>(op, op.fb.chs[c], op.fb.chs[c + 1], vb, lane, dig, ftab, tid, V[c], y);
+    // phase 2: channel 0 only, where most of the slow steps are
+    const int *g2 = op.fb.sgrp[2];
+    fb_slow<NS, TIn, TAcc, 1, 2, 0, 0>(op, xs, vl, rb, fw, fwm, g2[0], g2[1], V);
+    fb_slow<NS, TIn, TAcc, 1, 3, 0, 0>(op, xs, vl, rb, fw, fwm, g2[1], g2[2], V);
+    fb_slow<NS, TIn, TAcc, 1, 4, 0, 0>(op, xs, vl, rb, fw, fwm, g2[2], g2[3], V);
+    fb_slow<NS, TIn, TAcc, 1, 2, 2, 0>(op, xs, vl, rb, fw, fwm, g2[3], g2[4], V);
+    fb_slow<NS, TIn, TAcc, 1, 3, 2, 0>(op, xs, vl, rb, fw, fwm, g2[4], g2[5], V);
+    fb_slow<NS, TIn, TAcc, 1, 2, 3, 0>(op, xs, vl, rb, fw, fwm, g2[5], g2[6], V);
+    fb_slow<NS, TIn, TAcc, 1, 2, 2, 2>(op, xs, vl, rb, fw, fwm, g2[6], g2[7], V);
+    fb_slow<NS, TIn, TAcc, 1, 0, 0, 0>(op, xs, vl, rb, fw, fwm, g2[7], g2[8], V);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) y[s] += V[0][s];
     if (op.fb.npure > 0) {
         TAcc xr[NS];
 #pragma unroll
@@ -1346,11 +1361,11 @@ template <typename TAcc> __device__ __forceinline__ FbWave<TAcc> fb_prologue(con
 }
 
 // Iteration H*v in the fast-block form: y = offdiag(H) q_i, alpha partials.
-// Waves per SIMD the compiler must fit (register budget): left free, the
-// FP32 kernel holds 87 VGPRs (5 waves) and the FP64 one 132 (3 waves); at 8
-// and 4 waves they run 13.2 and 30.5 ms against 15.1 and 36.2 ms (four
-// states, 22M rows; FP32 at 7 waves: 13.8 ms; FP64 at 5: 30.3 ms, but it
-// spills 12 GB of scratch per launch).
+// Waves per SIMD the compiler must fit (register budget): 8 for FP32 (60
+// VGPRs) and 4 for FP64 (112).  Four states, 22M rows: FP32 10.7 ms at 6, 7
+// or 8 waves; FP64 23.6 ms at 3 or 4 waves and 28.0 ms at 5, where it spills.
+// (Before the slow monomials were split into phases, the ten channels held 40
+// VGPRs across all of them and the FP32 kernel waited on every load.)
 #ifndef NV_FB_WAVES32
 #define NV_FB_WAVES32 8
 #endif

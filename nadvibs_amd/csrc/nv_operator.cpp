@@ -406,11 +406,24 @@ static void build_fast_block(Operator &op, CodeOf &code_of, std::vector<std::arr
         S.nchk = std::max<int32_t>(S.nchk, ch->second + 1);
         if (ch->second == 0 && t.diag) S.excl0 = 1;
     }
-    for (int q = 0; q < NV_FB_NSLOW; ++q) {
-        op.fb_sgrp[q] = (int32_t)op.fb_S.size();
-        op.fb_S.insert(op.fb_S.end(), byshape[q].begin(), byshape[q].end());
+    // phases by the channels a slow monomial feeds: 0 beyond the first four (a fast monomial of
+    // degree 2: the slow one has degree <= 1, shapes (2) or ()), 1 within 1..3 (degree <= 2:
+    // (2), (3), (2,2) or ()), 2 only channel 0; each phase instantiates the shapes it can hold
+    static const bool kPhaseShape[3][NV_FB_NSLOW] = {{1, 0, 0, 0, 0, 0, 0, 1},
+                                                     {1, 1, 0, 1, 0, 0, 0, 1},
+                                                     {1, 1, 1, 1, 1, 1, 1, 1}};
+    for (int ph = 0; ph < 3; ++ph) {
+        for (int q = 0; q < NV_FB_NSLOW; ++q) {
+            op.fb_sgrp[ph][q] = (int32_t)op.fb_S.size();
+            for (auto &S : byshape[q]) {
+                const int sp_ = S.nchk > 4 ? 0 : (S.nchk > 1 ? 1 : 2);
+                if (sp_ != ph) continue;
+                if (!kPhaseShape[ph][q]) return;
+                op.fb_S.push_back(S);
+            }
+        }
+        op.fb_sgrp[ph][NV_FB_NSLOW] = (int32_t)op.fb_S.size();
     }
-    op.fb_sgrp[NV_FB_NSLOW] = (int32_t)op.fb_S.size();
     op.fb_nch = (int32_t)chkeys.size();
     op.fb_mf = mf;
     op.fb_P = (int32_t)P;

@@ -65,8 +65,8 @@ struct Operator {
     int32_t fb_mf = 0, fb_P = 0, fb_halo = 0, fb_pad = 0, fb_nch = 0;
     std::vector<int32_t> fb_tabmode;       // [ntab] mode of each factor row
     std::vector<double> fb_tab;            // [ntab][nmax][4]
-    std::vector<NvFbS> fb_S;               // grouped by slow shape
-    int32_t fb_sgrp[NV_FB_NSLOW + 1] = {0};
+    std::vector<NvFbS> fb_S;               // grouped by phase, then by slow shape
+    int32_t fb_sgrp[3][NV_FB_NSLOW + 1] = {};
     int32_t fb_chs[NV_FB_NCH + 1] = {0};
     std::vector<NvStep> fb_steps;          // fast steps of the channels, then of the pure terms
     std::vector<NvFbPure> fb_pure;
