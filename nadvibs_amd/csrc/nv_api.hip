@@ -1876,6 +1876,7 @@ struct nv_batch {
     std::vector<void *> allocs, run_allocs;
     size_t lds_hv = 0, lds_step = 0;
     hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph16 = nullptr;  // 16 iterations
     int bs = 256;
     std::vector<nv::BatchGroup> groups;
 };
@@ -1933,9 +1934,14 @@ hipError_t batch_tiles(nv_batch *b, int bs) {
     b->d.T = (int32_t)t.size();
     b->d.bs = bs;
     b->bs = bs;
+    // captured iterations name the tiles
     if (b->graph) {
         hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
+    }
+    if (b->graph16) {
+        hipGraphExecDestroy(b->graph16);
+        b->graph16 = nullptr;
     }
     return e;
 }
@@ -1954,6 +1960,10 @@ void batch_release_run(nv_batch *b) {
     if (b->graph) {
         hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
+    }
+    if (b->graph16) {
+        hipGraphExecDestroy(b->graph16);
+        b->graph16 = nullptr;
     }
     for (void *p : b->run_allocs) hipFree(p);
     b->run_allocs.clear();
@@ -2112,18 +2122,33 @@ int32_t nv_batch_run(nv_batch *b, int64_t iiter, int64_t niter) {
     if (iiter != b->next) return bfail(b, NV_ERR_STATE, "run must start at the next iteration");
     if (niter > b->niter) return bfail(b, NV_ERR_ARG, "niter beyond the initialised run");
     NV_BHIP(b, hipSetDevice(b->device));
+    // one captured iteration, or kBatchIters of them in one launch
+    constexpr int64_t kBatchIters = 16;
+    auto capture = [&](int64_t count, hipGraphExec_t *out) -> hipError_t {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t e2 = hipSuccess;
+        for (int64_t k = 0; k < count && e2 == hipSuccess; ++k) e2 = batch_enqueue(b, false, 0);
+        e = hipStreamEndCapture(b->stream, &g);
+        if (e2 != hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        if (g) hipGraphDestroy(g);
+        return e;
+    };
     for (int64_t i = iiter; i <= niter; ++i) {
         if (i == iiter) {
             NV_BHIP(b, batch_enqueue(b, true, i));
+        } else if (i + kBatchIters - 1 <= niter) {
+            if (!b->graph16) {
+                hipError_t e = capture(kBatchIters, &b->graph16);
+                if (e != hipSuccess) return bfail(b, NV_ERR_HIP, std::string("batch graph: ") + hipGetErrorString(e));
+            }
+            NV_BHIP(b, hipGraphLaunch(b->graph16, b->stream));
+            i += kBatchIters - 1;
         } else {
             if (!b->graph) {
-                hipGraph_t g = nullptr;
-                NV_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-                hipError_t e2 = batch_enqueue(b, false, 0);
-                hipError_t e = hipStreamEndCapture(b->stream, &g);
-                if (e2 != hipSuccess) e = e2;
-                if (e == hipSuccess) e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-                if (g) hipGraphDestroy(g);
+                hipError_t e = capture(1, &b->graph);
                 if (e != hipSuccess) return bfail(b, NV_ERR_HIP, std::string("batch graph: ") + hipGetErrorString(e));
             }
             NV_BHIP(b, hipGraphLaunch(b->graph, b->stream));
