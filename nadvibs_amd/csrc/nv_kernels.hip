@@ -303,6 +303,13 @@ constexpr int kBufFlags = 0x00020000;
 
 template <int NS, typename TIn> struct XRes;
 
+// One interleaved row as loaded (P = nv_lo_stride values), held apart from its
+// use so that a loop can keep loads of the next iteration in flight.
+template <int P> struct FRow { nv_f4 v[P / 4]; };
+template <> struct FRow<1> { float v; };
+template <> struct FRow<2> { nv_f2 v; };
+template <> struct FRow<3> { nv_f3 v; };
+
 // interleaved FP32 copy, x[r * nv_lo_stride(NS) + s]: one load per row up to
 // four states, 16-byte loads of the states present beyond
 template <int NS> struct XRes<NS, float> {
@@ -339,6 +346,30 @@ template <int NS> struct XRes<NS, float> {
 #pragma unroll
                 for (int s = part * 4; s < NS && s < part * 4 + 4; ++s) w[s] += h * (TAcc)v[s - part * 4];
             }
+        }
+    }
+    using Row = FRow<nv_lo_stride(NS)>;
+    __device__ __forceinline__ Row load_so(unsigned o, unsigned so) const {
+        constexpr int P = nv_lo_stride(NS);
+        Row x;
+        if constexpr (P == 1) x.v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)o, (int)so, 0));
+        else if constexpr (P == 2) x.v = __builtin_bit_cast(nv_f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o, (int)so, 0));
+        else if constexpr (P == 3) x.v = __builtin_bit_cast(nv_f3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)o, (int)so, 0));
+        else {
+#pragma unroll
+            for (int part = 0; part < P / 4; ++part)
+                x.v[part] = __builtin_bit_cast(
+                    nv_f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(o + 16u * part), (int)so, 0));
+        }
+        return x;
+    }
+    template <typename TAcc> static __device__ __forceinline__ void fma_row(const Row &x, TAcc h, TAcc (&w)[NS]) {
+        constexpr int P = nv_lo_stride(NS);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if constexpr (P == 1) w[s] += h * (TAcc)x.v;
+            else if constexpr (P <= 3) w[s] += h * (TAcc)x.v[s];
+            else w[s] += h * (TAcc)x.v[s / 4][s % 4];
         }
     }
 };
@@ -388,6 +419,30 @@ template <int NS> struct XRes<NS, F64I> {
                 w[2 * part] += h * (TAcc)v;
             }
         }
+    }
+    typedef double nv_rd2 __attribute__((ext_vector_type(2)));
+    struct Row {
+        nv_rd2 v[NS / 2];
+        double t;  // the last state when NS is odd
+    };
+    __device__ __forceinline__ Row load_so(unsigned o, unsigned so) const {
+        Row x;
+#pragma unroll
+        for (int part = 0; 2 * part + 1 < NS; ++part)
+            x.v[part] = __builtin_bit_cast(nv_rd2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(o + 16u * part), (int)so, 0));
+        if constexpr (NS % 2 == 1)
+            x.t = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(o + 16u * (NS / 2)), (int)so, 0));
+        else
+            x.t = 0.0;
+        return x;
+    }
+    template <typename TAcc> static __device__ __forceinline__ void fma_row(const Row &x, TAcc h, TAcc (&w)[NS]) {
+#pragma unroll
+        for (int s = 0; s + 1 < NS; s += 2) {
+            w[s] += h * (TAcc)x.v[s / 2][0];
+            w[s + 1] += h * (TAcc)x.v[s / 2][1];
+        }
+        if constexpr (NS % 2 == 1) w[NS - 1] += h * (TAcc)x.t;
     }
 };
 
@@ -1114,78 +1169,112 @@ __device__ __forceinline__ void fb_mix(const TAcc *__restrict__ cm, const NvFbS 
 // The slow monomials [t0, t1) of shape (K0, K1, K2), gathered and mixed into
 // the channels V; vl is the lane's byte offset in its block, rb the byte
 // offset of the block's first row in the padded buffer, fw/fwm the wave's
-// factor rows and their masks of slots inside the basis.
+// factor rows and their masks of slots inside the basis.  The loop is
+// software-pipelined: monomial t+1's rows are loaded while monomial t's are
+// summed and mixed, each load reusing the registers of the row just summed,
+// so a wave keeps one monomial's loads in flight through the mixing.
 template <int NS, typename TIn, typename TAcc, int NC, int K0, int K1, int K2>
 __device__ __forceinline__ void fb_slow(const NvOpDev &op, const XRes<NS, TIn> &xs, unsigned vl, unsigned rb,
                                         const TAcc *fw, const int *fwm, int t0, int t1,
                                         TAcc (&V)[NV_FB_NCH][NS]) {
-    constexpr unsigned kEl = XRes<NS, TIn>::kEl;
-    constexpr int J0 = K0 ? K0 : 1, J1 = K1 ? K1 : 1, J2 = K2 ? K2 : 1;
+    using X = XRes<NS, TIn>;
+    constexpr unsigned kEl = X::kEl;
+    constexpr int J0 = K0 ? K0 : 1, J1 = K1 ? K1 : 1, J2 = K2 ? K2 : 1, J = J0 * J1 * J2;
     // all stepping slots odd: the middle step is Delta_s = 0
     constexpr bool odd = K0 > 0 && (K0 & 1) && (K1 == 0 || (K1 & 1)) && (K2 == 0 || (K2 & 1));
+    if (t0 >= t1) return;
     const TAcc *__restrict__ cm = cmat_of<TAcc>(op);
-    for (int t = t0; t < t1; ++t) {
-        const NvFbS d = op.fb.S[t];
-        Fb4<TAcc> f0, f1, f2;
+    // scalar byte offsets of monomial t's steps, in (i0, i1, i2) order; a step
+    // outside the basis reads the zero margin at offset 0
+    auto offsets = [&](int t, unsigned (&so)[J]) {
+        const NvFbS &d = op.fb.S[t];
         // slot i of slow slot j is inside the basis when bit i of m_j is set
         int m0 = 1, m1 = 1, m2 = 1;
-        if constexpr (K0 > 0) {
-            f0 = *reinterpret_cast<const Fb4<TAcc> *>(fw + d.tab[0] * 4);
-            m0 = __builtin_amdgcn_readfirstlane(fwm[d.tab[0]]);
-        }
-        if constexpr (K1 > 0) {
-            f1 = *reinterpret_cast<const Fb4<TAcc> *>(fw + d.tab[1] * 4);
-            m1 = __builtin_amdgcn_readfirstlane(fwm[d.tab[1]]);
-        }
-        if constexpr (K2 > 0) {
-            f2 = *reinterpret_cast<const Fb4<TAcc> *>(fw + d.tab[2] * 4);
-            m2 = __builtin_amdgcn_readfirstlane(fwm[d.tab[2]]);
-        }
+        if constexpr (K0 > 0) m0 = __builtin_amdgcn_readfirstlane(fwm[d.tab[0]]);
+        if constexpr (K1 > 0) m1 = __builtin_amdgcn_readfirstlane(fwm[d.tab[1]]);
+        if constexpr (K2 > 0) m2 = __builtin_amdgcn_readfirstlane(fwm[d.tab[2]]);
         const unsigned b0 = rb + (unsigned)d.base * kEl;
         const unsigned ib0 = (unsigned)d.inc[0] * kEl, ib1 = (unsigned)d.inc[1] * kEl,
                        ib2 = (unsigned)d.inc[2] * kEl;
-        // w: every step but the middle one of an all-odd shape, wc: that one
-        TAcc w[NS], wc[NS];
-#pragma unroll
-        for (int s = 0; s < NS; ++s) w[s] = wc[s] = (TAcc)0;
 #pragma unroll
         for (int i0 = 0; i0 < J0; ++i0)
 #pragma unroll
             for (int i1 = 0; i1 < J1; ++i1)
 #pragma unroll
                 for (int i2 = 0; i2 < J2; ++i2) {
-                    TAcc h = (TAcc)1;
-                    unsigned so = rb;
-                    if constexpr (K0 > 0) {
-                        h = f0.v[i0];
-                        so = b0 + (unsigned)i0 * ib0;
-                    }
-                    if constexpr (K1 > 0) {
-                        h *= f1.v[i1];
-                        so += (unsigned)i1 * ib1;
-                    }
-                    if constexpr (K2 > 0) {
-                        h *= f2.v[i2];
-                        so += (unsigned)i2 * ib2;
-                    }
+                    unsigned o = rb;
+                    if constexpr (K0 > 0) o = b0 + (unsigned)i0 * ib0;
+                    if constexpr (K1 > 0) o += (unsigned)i1 * ib1;
+                    if constexpr (K2 > 0) o += (unsigned)i2 * ib2;
                     const bool in = ((m0 >> i0) & (K1 > 0 ? m1 >> i1 : 1) & (K2 > 0 ? m2 >> i2 : 1) & 1) != 0;
-                    const bool mid =
-                        odd && 2 * i0 + 1 == K0 && (K1 == 0 || 2 * i1 + 1 == K1) && (K2 == 0 || 2 * i2 + 1 == K2);
-                    if (mid) xs.template fma_so<TAcc>(vl, in ? so : 0u, h, wc);
-                    else xs.template fma_so<TAcc>(vl, in ? so : 0u, h, w);
+                    so[(i0 * J1 + i1) * J2 + i2] = in ? o : 0u;
                 }
+    };
+    typename X::Row xr[J];
+    {
+        unsigned so[J];
+        offsets(t0, so);
+#pragma unroll
+        for (int k = 0; k < J; ++k) {
+            xr[k] = xs.load_so(vl, so[k]);
+            // loads leave in step order, the order the loop consumes them
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    for (int t = t0; t < t1; ++t) {
+        const NvFbS &d = op.fb.S[t];
+        Fb4<TAcc> f0, f1, f2;
+        if constexpr (K0 > 0) f0 = *reinterpret_cast<const Fb4<TAcc> *>(fw + d.tab[0] * 4);
+        if constexpr (K1 > 0) f1 = *reinterpret_cast<const Fb4<TAcc> *>(fw + d.tab[1] * 4);
+        if constexpr (K2 > 0) f2 = *reinterpret_cast<const Fb4<TAcc> *>(fw + d.tab[2] * 4);
+        // w: every step but the middle one of an all-odd shape, wc: that one
+        TAcc w[NS], wc[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) w[s] = wc[s] = (TAcc)0;
+        // sums the rows in flight; with next, each row's registers then take
+        // the next monomial's row
+        auto sum_rows = [&](auto next, const unsigned (&sn)[J]) {
+#pragma unroll
+            for (int i0 = 0; i0 < J0; ++i0)
+#pragma unroll
+                for (int i1 = 0; i1 < J1; ++i1)
+#pragma unroll
+                    for (int i2 = 0; i2 < J2; ++i2) {
+                        const int k = (i0 * J1 + i1) * J2 + i2;
+                        TAcc h = (TAcc)1;
+                        if constexpr (K0 > 0) h = f0.v[i0];
+                        if constexpr (K1 > 0) h *= f1.v[i1];
+                        if constexpr (K2 > 0) h *= f2.v[i2];
+                        const bool mid = odd && 2 * i0 + 1 == K0 && (K1 == 0 || 2 * i1 + 1 == K1) &&
+                                         (K2 == 0 || 2 * i2 + 1 == K2);
+                        if (mid) X::template fma_row<TAcc>(xr[k], h, wc);
+                        else X::template fma_row<TAcc>(xr[k], h, w);
+                        if constexpr (decltype(next)::value) {
+                            xr[k] = xs.load_so(vl, sn[k]);
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+        };
+        unsigned sn[J];
+        if (t + 1 < t1) {
+            offsets(t + 1, sn);
+            sum_rows(std::true_type{}, sn);
+        } else {
+            sum_rows(std::false_type{}, sn);
+        }
         if (d.nd > 0) {
             TAcc fd = fw[d.dtab[0] * 4];
             if (d.nd > 1) fd *= fw[d.dtab[1] * 4];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 w[s] *= fd;
-                wc[s] *= fd;
+                if constexpr (odd) wc[s] *= fd;
             }
         }
+        // wc stays zero unless the shape has a middle step
         TAcc wf[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) wf[s] = w[s] + wc[s];
+        for (int s = 0; s < NS; ++s) wf[s] = odd ? w[s] + wc[s] : w[s];
         fb_mix<NS, TAcc, NC, odd>(cm, d, w, wf, V);
     }
 }
