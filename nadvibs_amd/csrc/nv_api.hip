@@ -1875,6 +1875,7 @@ struct nv_batch {
     int32_t *tile0_d = nullptr;
     std::vector<void *> allocs, run_allocs;
     size_t lds_hv = 0, lds_step = 0;
+    bool rows4 = false;  // lds_hv holds the digits of four rows per thread (FP32 shaped H*v)
     hipGraphExec_t graph = nullptr;
     hipGraphExec_t graph16 = nullptr;  // 16 iterations
     int bs = 256;
@@ -2111,7 +2112,23 @@ int32_t nv_batch_init(nv_batch *b, int64_t niter, int32_t prec) {
     b->last = 0;
     b->next = 1;
     b->lds_hv = 0;
-    for (nv_ctx *c : b->ctx) b->lds_hv = std::max(b->lds_hv, nv::hv_lds_bytes(c->dev, prec));
+    size_t lds4 = 0;
+    for (nv_ctx *c : b->ctx) {
+        const size_t l = nv::hv_lds_bytes(c->dev, prec);
+        b->lds_hv = std::max(b->lds_hv, l);
+        lds4 = std::max(lds4, l + 2 * (size_t)c->dev.M * 256);  // two more rows' digits per thread
+    }
+    // four rows per thread (FP32, shaped form) where their digits fit in 64 KB of LDS
+    b->rows4 = prec != NV_PREC_F64 && b->d.shaped[1] != 0 && lds4 <= 65536;
+    if (b->rows4) b->lds_hv = lds4;
+    if (const char *r = std::getenv("NV_BATCH_ROWS")) {
+        const int v = std::atoi(r);
+        if (v != 1 && v != 2 && v != 4) return bfail(b, NV_ERR_ARG, "NV_BATCH_ROWS must be 1, 2 or 4");
+        if (v > 1 && prec == NV_PREC_F64) return bfail(b, NV_ERR_ARG, "NV_BATCH_ROWS above 1 needs FP32 H*v");
+        if (v == 4 && (!b->rows4 || b->bs < 256))
+            return bfail(b, NV_ERR_ARG, "four rows per thread need the shaped form, 256-row tiles and LDS room");
+        b->d.rows = v;
+    }
     b->ready = true;
     return NV_OK;
 }
@@ -2190,8 +2207,9 @@ int32_t nv_batch_tune(nv_batch *b, char *report, int64_t len) {
     for (int steps = 0; steps <= 1; ++steps) {
         if (steps == 0 && !shaped) continue;
         for (int bs : {64, 128, 256})
-            for (int rows = 1; rows <= 2; ++rows) {
+            for (int rows : {1, 2, 4}) {
                 if (rows == 2 && (b->prec == NV_PREC_F64 || bs < 128)) continue;
+                if (rows == 4 && (steps || !b->rows4 || bs < 256)) continue;
                 cands.push_back({bs, rows, steps});
             }
     }

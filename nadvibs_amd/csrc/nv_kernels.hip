@@ -2672,6 +2672,9 @@ static void batch_hv_ns(const BatchDev &b, int prec, size_t lds, int t0, int nt,
         if (b.shaped[prec == 0 ? 0 : 1] && !b.steps) {
             if (prec == 0)
                 hipLaunchKernelGGL((batch_shaped_kernel<NS, double, double, 1>), dim3(nt), dim3(b.bs), lds, s, b, t0);
+            else if (b.rows == 4 && b.bs >= 256)
+                hipLaunchKernelGGL((batch_shaped_kernel<NS, float, float, 4>), dim3(nt), dim3(b.bs / 4), lds, s, b,
+                                   t0);
             else if (two)
                 hipLaunchKernelGGL((batch_shaped_kernel<NS, float, float, 2>), dim3(nt), dim3(b.bs / 2), lds, s, b,
                                    t0);

@@ -170,7 +170,8 @@ struct BatchDev {
     const int64_t *iter = nullptr;    // current iteration (device)
     int32_t P = 0, T = 0, bs = 256;   // problems, tiles, rows per tile (= workgroup size)
     int32_t shaped[2] = {0, 0};       // every operator runs the shaped H*v (FP64, FP32)
-    int32_t rows = 2;                 // FP32 H*v: rows per thread (1 or 2; 2 needs bs >= 128)
+    int32_t rows = 2;                 // FP32 H*v: rows per thread (1, 2 or 4; 2 needs bs >= 128, 4 the shaped
+                                      // form, bs >= 256 and launch LDS for four rows' digits)
     int32_t steps = 0;                // 1: the step-table H*v even where the shaped one serves
 };
 // Tiles [t0, t0+nt) all belong to problems with ns states (one launch each).

@@ -52,6 +52,31 @@ def test_batch_matches_single_runs(prec, tol, form, monkeypatch):
         np.testing.assert_allclose(b, b1, rtol=tol, atol=1e-15)
 
 
+@pytest.mark.parametrize("rows", ["1", "4"])
+def test_batch_rows_per_thread_match_single_runs(rows, monkeypatch):
+    """One and four rows per thread of the FP32 shaped batched H*v (the
+    default is two) give the single-problem coefficients."""
+    monkeypatch.setenv("NV_BATCH_ROWS", rows)
+    monkeypatch.delenv("NV_HV_KERNEL", raising=False)
+    probs = _probs()
+    n = 40
+    with Batch(probs) as bt:
+        bt.init(n, "f32")
+        bt.run()
+        got = [bt.coefficients(i) for i in range(len(probs))]
+    for p, (a, b) in zip(probs, got):
+        a1, b1 = _single(p, n, "f32")
+        np.testing.assert_allclose(a, a1, rtol=1e-8, atol=1e-15)
+        np.testing.assert_allclose(b, b1, rtol=1e-8, atol=1e-15)
+
+
+def test_batch_rows_setting_is_checked(monkeypatch):
+    monkeypatch.setenv("NV_BATCH_ROWS", "3")
+    with Batch(_probs()[:2]) as bt:
+        with pytest.raises(Exception, match="NV_BATCH_ROWS"):
+            bt.init(10, "f32")
+
+
 def test_batch_matches_oracle_and_tuning_keeps_results():
     probs = _probs()
     n = 30
