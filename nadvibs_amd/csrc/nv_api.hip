@@ -684,7 +684,8 @@ int32_t nv_lanczos_init(nv_ctx *c, const nv_lanczos_opts *o) {
         if (c->dev.fb.on) nparts = std::max<int64_t>(nparts, nv::hv_fb_parts(c->dev) + 64);
         NV_HIP(c, alloc(nparts * sizeof(double), (void **)&st.partials));
         NV_HIP(c, alloc(nv::kNumCounters * sizeof(unsigned), (void **)&st.counters));
-        NV_HIP(c, alloc(sizeof(int64_t), (void **)&c->d_iter));
+        // the iteration counter, and a second one that folded graph iterations alternate with
+        NV_HIP(c, alloc(2 * sizeof(int64_t), (void **)&c->d_iter));
         st.iter = c->d_iter;
         if (K > 0) {
             c->ncp = (o->niter + K - 1) / K;
@@ -923,6 +924,60 @@ static hipError_t enqueue_iteration(nv_ctx *c, int pro, int check, bool set = tr
 // iteration on a 1.38M-element operator with one iteration per graph.
 constexpr int64_t kIterBatch = 16;
 
+// Up to kFoldMax partials per sum, graph kind 3 leaves out the two finishing
+// kernels of an iteration: the step kernel sums the H*v's alpha partials
+// itself (every workgroup, in the finishing kernel's order) and stores alpha,
+// the next iteration's normalisation sums the step's beta partials and stores
+// beta.  The iteration counter then cannot advance between kernels, so
+// iterations alternate between two counters: the step of iteration k sets the
+// counter that iteration k+1 reads (c->d_iter for even k, c->d_iter + 1 for
+// odd).  The first iteration of a launch normalises with the stored beta, and
+// the last one's beta goes through the finishing kernel, so a launch starts
+// and ends in the state the per-call path leaves.
+constexpr int kFoldMax = 4096;
+
+// The iteration H*v without its alpha finish, and the partials it writes.
+static hipError_t launch_hv_iter_nofinish(nv_ctx *c, const nv::LzState &st) {
+    const nv::JitKernels *k = ensure_jit(c, c->opts.prec);
+    if (k) return nv::jit_launch_iter(*k, c->dev, st, c->stream);
+    return nv::launch_hv_iter_only(c->dev, c->opts.prec, st, c->stream);
+}
+static int hv_iter_nparts(nv_ctx *c) {
+    return ensure_jit(c, c->opts.prec) ? nv::hv_blocks(c->dev) : nv::hv_iter_parts(c->dev, c->opts.prec, c->st);
+}
+static bool fold_ok(nv_ctx *c) {
+    return hv_iter_nparts(c) <= kFoldMax && nv::hv_blocks(c->dev) <= kFoldMax;
+}
+
+static hipError_t enqueue_folded_batch(nv_ctx *c) {
+    hipStream_t s = c->stream;
+    int64_t *ctr[2] = {c->d_iter, c->d_iter + 1};
+    double *bparts = c->st.partials + kFoldMax;  // the partials buffer holds at least 8192 + 64
+    const int nhv = hv_iter_nparts(c), nstep = nv::hv_blocks(c->dev);
+    hipError_t e = nv::launch_set_iter(ctr[0], 0, 1, s);
+    for (int64_t k = 0; k < kIterBatch && e == hipSuccess; ++k) {
+        nv::LzState st = c->st;
+        st.iter = ctr[k & 1];
+        nv::LzState sn = st;
+        if (k > 0) {
+            sn.fold_in = bparts;
+            sn.fold_n = nstep;
+        }
+        e = nv::launch_normalize(c->dev, c->opts.prec, sn, s);
+        if (e == hipSuccess) e = launch_hv_iter_nofinish(c, st);
+        const bool last = k + 1 == kIterBatch;
+        nv::LzState ss = st;
+        ss.fold_in = st.partials;
+        ss.fold_n = nhv;
+        ss.fold_out = bparts;
+        ss.next_iter = last ? ctr[0] : ctr[(k + 1) & 1];
+        ss.next_inc = last ? 0 : 1;
+        if (e == hipSuccess) e = nv::launch_lanczos_step_only(c->dev, c->opts.prec, ss, s);
+        if (e == hipSuccess && last) e = nv::launch_finish_sum(bparts, nstep, c->st.beta, st.iter, 1, s);
+    }
+    return e;
+}
+
 static hipError_t iteration_graph(nv_ctx *c, int kind, hipGraphExec_t *out) {
     if (c->graphs[kind]) {
         *out = c->graphs[kind];
@@ -932,7 +987,9 @@ static hipError_t iteration_graph(nv_ctx *c, int kind, hipGraphExec_t *out) {
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
     hipError_t e2 = hipSuccess;
-    if (kind == 3)
+    if (kind == 3 && fold_ok(c))
+        e2 = enqueue_folded_batch(c);
+    else if (kind == 3)
         for (int64_t k = 0; k < kIterBatch && e2 == hipSuccess; ++k)
             e2 = enqueue_iteration(c, 0, 0, k == 0, k + 1 < kIterBatch);
     else

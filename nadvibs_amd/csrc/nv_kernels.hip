@@ -1534,6 +1534,15 @@ __global__ __launch_bounds__(kBlock) void hv_fb_full_kernel(NvOpDev op, const vo
     }
 }
 
+// The finishing kernel's sum of parts[0, n) (finish_sum_kernel: strided
+// per-thread sums, then the block tree), in every workgroup of kBlock threads.
+__device__ __forceinline__ double fold_sum(const double *parts, int n) {
+    double s = 0.0;
+#pragma unroll 8
+    for (int i = threadIdx.x; i < n; i += kBlock) s += parts[i];
+    return block_sum(s);
+}
+
 // lanczos_step (Source/top.f90:820-842): r = (H q_i - alpha q_i) - beta(i-1) q_{i-1}
 // stored in slot i+1, beta(i) = |r|.
 template <int NS, typename TAcc>
@@ -1545,7 +1554,13 @@ __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzStat
     const double *q = st.V + slot_of(it, st.nslots) * (int64_t)op.N;
     const double *qm = st.V + slot_of(it - 1, st.nslots) * (int64_t)op.N;
     double *res = st.V + slot_of(it + 1, st.nslots) * (int64_t)op.N;
-    const double a = st.alpha[it];
+    double a;
+    if (st.fold_in) {
+        a = fold_sum(st.fold_in, st.fold_n);
+        if (blockIdx.x == 0 && tid == 0) st.alpha[it] = a;
+    } else {
+        a = st.alpha[it];
+    }
     const double b = it > 1 ? st.beta[it - 1] : 0.0;
     double part = 0.0;
     if (r < op.D) {
@@ -1566,7 +1581,8 @@ __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzStat
         }
     }
     const double bs = block_sum(part);
-    if (tid == 0) st.partials[blockIdx.x] = bs;
+    if (tid == 0) (st.fold_out ? st.fold_out : st.partials)[blockIdx.x] = bs;
+    if (st.next_iter && blockIdx.x == 0 && tid == 0) *st.next_iter = it + st.next_inc;
 }
 
 
@@ -1586,7 +1602,14 @@ __global__ __launch_bounds__(kBlock) void normalize_rows_kernel(int64_t N, int64
     constexpr int NSP = nv_lo_stride(NS);
     const int64_t it = *st.iter;
     double *q = st.V + slot_of(it, st.nslots) * N;
-    const double sc = 1.0 / st.beta[it - 1];
+    double bprev;
+    if (st.fold_in) {
+        bprev = sqrt(fold_sum(st.fold_in, st.fold_n));
+        if (blockIdx.x == 0 && threadIdx.x == 0) st.beta[it - 1] = bprev;
+    } else {
+        bprev = st.beta[it - 1];
+    }
+    const double sc = 1.0 / bprev;
     TIn *lo = reinterpret_cast<TIn *>(st.vlo);
     float *vf = st.Vf ? st.Vf + slot_of(it, st.nslots) * st.ldf : nullptr;
     for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < D; r += (int64_t)gridDim.x * kBlock) {

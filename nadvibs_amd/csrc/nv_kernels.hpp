@@ -30,6 +30,15 @@ struct LzState {
     double *v64i = nullptr;      // FP64 runs of the shaped generic H*v: q_i interleaved
                                  // [D][nv_lo_stride(ns)], padding zero, written by normalize
     void *z2 = nullptr;          // channel scratch of the two-pass H*v (hv2_scratch_bytes), or null
+    // Finishing sums folded into their consumers (iterations without re-orthogonalisation on
+    // operators with few partials): every workgroup sums fold_in[0, fold_n) in the finishing
+    // kernel's order, so the result has the same bits.
+    const double *fold_in = nullptr;  // normalize: the previous step's beta partials (block 0 stores
+                                      // beta[it-1]); step: this H*v's alpha partials (stores alpha[it])
+    int32_t fold_n = 0;
+    double *fold_out = nullptr;       // step: its beta partials go here instead of partials
+    int64_t *next_iter = nullptr;     // step: block 0 stores it + next_inc there (a counter that no
+    int32_t next_inc = 0;             //   workgroup of the same launch reads)
 };
 
 // counters[] slots
