@@ -50,6 +50,21 @@ def test_invalid_problem_is_rejected(over, msg):
     assert msg in lib.nv_last_error(None).decode()
 
 
+@pytest.mark.parametrize("ns,nfunc,msg", [(2, [2] * 31, "2^31"), (1, [2] * 101, "nmodes")])
+def test_size_limits_are_refused_before_any_device_call(ns, nfunc, msg):
+    """nstates * dimen from 2^31 on (here 2 x 2^31) and more than 100 modes (the
+    reference's basis(100)) are refused with a message; 100 modes pass the check
+    (the GPU tests run a 100-mode operator)."""
+    lib = _lib.load()
+    nm = len(nfunc)
+    p = make_problem(ns, nfunc, 1, (500, 900), 10.0, 0.0, 0.0, seed=1)
+    desc, keep = _desc(p)
+    h = ctypes.c_void_p()
+    rc = lib.nv_create(ctypes.byref(desc), 0, ctypes.byref(h))
+    assert rc == 1 and not h.value, nm
+    assert msg in lib.nv_last_error(None).decode()
+
+
 def test_null_arguments():
     lib = _lib.load()
     h = ctypes.c_void_p()
