@@ -1489,15 +1489,19 @@ hv_fb_iter_kernel(NvOpDev op, LzState st) {
     if (w.inblk) fb_offdiag<NS, TIn, TAcc>(op, xs, vl, rb, w.lane, w.dig, w.fw, w.fwm, w.ftab, tid, w.vb, y);
     double part = 0.0;
     if (w.live) {
+        // q_i, dg and y are touched once per launch: non-temporal, so that they do not
+        // displace the rows of x that neighbouring blocks gather from L2
         const int r = w.row0 + w.lane;
         double v[NS], dv[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
-        diag_apply<NS>(op, r, w.dig, tid, v, dv);
+        for (int s = 0; s < NS; ++s) v[s] = __builtin_nontemporal_load(&q[(size_t)s * op.D + r]);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) dv[s] = __builtin_nontemporal_load(&op.dg[(size_t)s * op.D + r]) * v[s];
+        if (op.offstate) diag_offstate<NS>(op, w.dig, tid, v, dv);
         TAcc *yo = reinterpret_cast<TAcc *>(st.y);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            yo[(size_t)s * op.D + r] = y[s];
+            __builtin_nontemporal_store(y[s], &yo[(size_t)s * op.D + r]);
             part += v[s] * (dv[s] + (double)y[s]);
         }
     }
@@ -1566,17 +1570,20 @@ __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzStat
     if (r < op.D) {
         if (op.offstate) digits_energy(op, r, dig, tid);
         double v[NS], dv[NS];
+        // every vector is streamed once: non-temporal loads and stores
 #pragma unroll
-        for (int s = 0; s < NS; ++s) v[s] = q[(size_t)s * op.D + r];
-        diag_apply<NS>(op, r, dig, tid, v, dv);
+        for (int s = 0; s < NS; ++s) v[s] = __builtin_nontemporal_load(&q[(size_t)s * op.D + r]);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) dv[s] = __builtin_nontemporal_load(&op.dg[(size_t)s * op.D + r]) * v[s];
+        if (op.offstate) diag_offstate<NS>(op, dig, tid, v, dv);
         const TAcc *y = reinterpret_cast<const TAcc *>(st.y);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const size_t o = (size_t)s * op.D + r;
-            double w = dv[s] + (double)y[o];
+            double w = dv[s] + (double)__builtin_nontemporal_load(&y[o]);
             w = w - a * v[s];
-            if (it > 1) w = w - b * qm[o];
-            res[o] = w;
+            if (it > 1) w = w - b * __builtin_nontemporal_load(&qm[o]);
+            __builtin_nontemporal_store(w, &res[o]);
             part += w * w;
         }
     }
@@ -1616,8 +1623,8 @@ __global__ __launch_bounds__(kBlock) void normalize_rows_kernel(int64_t N, int64
         double v[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            v[s] = q[s * D + r] * sc;
-            q[s * D + r] = v[s];
+            v[s] = __builtin_nontemporal_load(&q[s * D + r]) * sc;
+            __builtin_nontemporal_store(v[s], &q[s * D + r]);
             if (vf) vf[s * D + r] = (float)v[s];
         }
         if (lo) {
