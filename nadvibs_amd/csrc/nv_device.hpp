@@ -180,4 +180,6 @@ struct NvOpDev {
     int64_t hv2_min_bytes;          // vector size from which the two-pass H*v serves the iteration
     NvFbDev fb;                     // fast-block H*v tables (fb.on = 0: not available)
     int64_t fb_min_bytes;           // vector size from which the fast-block H*v serves
+    int32_t stream_nt;              // vectors far beyond the caches: the step, the normalisation and
+                                    // the fast-block H*v's epilogue stream them with non-temporal accesses
 };

@@ -574,6 +574,18 @@ __global__ __launch_bounds__(kBlock) void hv_full_kernel(NvOpDev op, const doubl
 
 __device__ __forceinline__ int64_t slot_of(int64_t i, int64_t nslots) { return i % nslots; }
 
+// A vector element touched once per launch: non-temporal (NT) where the
+// vectors are far larger than the caches, plain otherwise (a vector that fits
+// the Infinity Cache is read again from it by the next kernel).
+template <bool NT, typename T> __device__ __forceinline__ T ld_once(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T> __device__ __forceinline__ void st_once(T v, T *p) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 // Iteration H*v: y = offdiag(H) q_i in TAcc, alpha(i) = q_i . (H q_i) in FP64.
 template <int NS, typename TIn, typename TAcc>
 __global__ __launch_bounds__(kBlock) void hv_iter_kernel(NvOpDev op, LzState st) {
@@ -1488,22 +1500,27 @@ hv_fb_iter_kernel(NvOpDev op, LzState st) {
     const unsigned rb = (unsigned)(w.row0 + op.fb.pad) * kEl;
     if (w.inblk) fb_offdiag<NS, TIn, TAcc>(op, xs, vl, rb, w.lane, w.dig, w.fw, w.fwm, w.ftab, tid, w.vb, y);
     double part = 0.0;
-    if (w.live) {
-        // q_i, dg and y are touched once per launch: non-temporal, so that they do not
-        // displace the rows of x that neighbouring blocks gather from L2
+    // q_i, dg and y are touched once per launch (non-temporal on the largest vectors, so that
+    // they do not displace the rows of x that neighbouring blocks gather from L2)
+    auto epilogue = [&](auto nt) {
+        constexpr bool NT = decltype(nt)::value;
         const int r = w.row0 + w.lane;
         double v[NS], dv[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) v[s] = __builtin_nontemporal_load(&q[(size_t)s * op.D + r]);
+        for (int s = 0; s < NS; ++s) v[s] = ld_once<NT>(&q[(size_t)s * op.D + r]);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) dv[s] = __builtin_nontemporal_load(&op.dg[(size_t)s * op.D + r]) * v[s];
+        for (int s = 0; s < NS; ++s) dv[s] = ld_once<NT>(&op.dg[(size_t)s * op.D + r]) * v[s];
         if (op.offstate) diag_offstate<NS>(op, w.dig, tid, v, dv);
         TAcc *yo = reinterpret_cast<TAcc *>(st.y);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            __builtin_nontemporal_store(y[s], &yo[(size_t)s * op.D + r]);
+            st_once<NT>(y[s], &yo[(size_t)s * op.D + r]);
             part += v[s] * (dv[s] + (double)y[s]);
         }
+    };
+    if (w.live) {
+        if (op.stream_nt) epilogue(std::true_type{});
+        else epilogue(std::false_type{});
     }
     const double bs = block_sum(part);
     if (tid == 0) st.partials[b] = bs;
@@ -1549,7 +1566,7 @@ __device__ __forceinline__ double fold_sum(const double *parts, int n) {
 
 // lanczos_step (Source/top.f90:820-842): r = (H q_i - alpha q_i) - beta(i-1) q_{i-1}
 // stored in slot i+1, beta(i) = |r|.
-template <int NS, typename TAcc>
+template <int NS, typename TAcc, bool NT>
 __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzState st) {
     extern __shared__ __align__(16) unsigned char dig[];  // M*kBlock bytes when states couple through D
     const int tid = threadIdx.x;
@@ -1570,20 +1587,20 @@ __global__ __launch_bounds__(kBlock) void lanczos_step_kernel(NvOpDev op, LzStat
     if (r < op.D) {
         if (op.offstate) digits_energy(op, r, dig, tid);
         double v[NS], dv[NS];
-        // every vector is streamed once: non-temporal loads and stores
+        // every vector is streamed once
 #pragma unroll
-        for (int s = 0; s < NS; ++s) v[s] = __builtin_nontemporal_load(&q[(size_t)s * op.D + r]);
+        for (int s = 0; s < NS; ++s) v[s] = ld_once<NT>(&q[(size_t)s * op.D + r]);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) dv[s] = __builtin_nontemporal_load(&op.dg[(size_t)s * op.D + r]) * v[s];
+        for (int s = 0; s < NS; ++s) dv[s] = ld_once<NT>(&op.dg[(size_t)s * op.D + r]) * v[s];
         if (op.offstate) diag_offstate<NS>(op, dig, tid, v, dv);
         const TAcc *y = reinterpret_cast<const TAcc *>(st.y);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const size_t o = (size_t)s * op.D + r;
-            double w = dv[s] + (double)__builtin_nontemporal_load(&y[o]);
+            double w = dv[s] + (double)ld_once<NT>(&y[o]);
             w = w - a * v[s];
-            if (it > 1) w = w - b * __builtin_nontemporal_load(&qm[o]);
-            __builtin_nontemporal_store(w, &res[o]);
+            if (it > 1) w = w - b * ld_once<NT>(&qm[o]);
+            st_once<NT>(w, &res[o]);
             part += w * w;
         }
     }
@@ -1604,7 +1621,7 @@ template <typename T, int K> struct alignas(il_align(sizeof(T) * K) < alignof(T)
 // update_lanczos (Source/top.f90:646-670): q_i = r / beta(i-1) in place, and
 // the copies of q_i that H*v and re-orthogonalisation read.  A thread takes
 // the NS states of row r, so an interleaved copy gets one row store.
-template <int NS, typename TIn>
+template <int NS, typename TIn, bool NT>
 __global__ __launch_bounds__(kBlock) void normalize_rows_kernel(int64_t N, int64_t D, LzState st) {
     constexpr int NSP = nv_lo_stride(NS);
     const int64_t it = *st.iter;
@@ -1623,8 +1640,8 @@ __global__ __launch_bounds__(kBlock) void normalize_rows_kernel(int64_t N, int64
         double v[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            v[s] = __builtin_nontemporal_load(&q[s * D + r]) * sc;
-            __builtin_nontemporal_store(v[s], &q[s * D + r]);
+            v[s] = ld_once<NT>(&q[s * D + r]) * sc;
+            st_once<NT>(v[s], &q[s * D + r]);
             if (vf) vf[s * D + r] = (float)v[s];
         }
         if (lo) {
@@ -1908,10 +1925,14 @@ static hipError_t hv2_ns(const NvOpDev &op, int prec, const void *xin, void *Z, 
 template <int NS>
 static hipError_t step_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     const size_t lds = op.offstate ? (size_t)op.M * kBlock : 0;
-    if (prec == 0)
-        hipLaunchKernelGGL((lanczos_step_kernel<NS, double>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
+    if (prec == 0 && op.stream_nt)
+        hipLaunchKernelGGL((lanczos_step_kernel<NS, double, true>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
+    else if (prec == 0)
+        hipLaunchKernelGGL((lanczos_step_kernel<NS, double, false>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
+    else if (op.stream_nt)
+        hipLaunchKernelGGL((lanczos_step_kernel<NS, float, true>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
     else
-        hipLaunchKernelGGL((lanczos_step_kernel<NS, float>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
+        hipLaunchKernelGGL((lanczos_step_kernel<NS, float, false>), dim3(hv_blocks(op)), dim3(kBlock), lds, s, op, st);
     return hipGetLastError();
 }
 
@@ -2045,15 +2066,24 @@ hipError_t launch_lanczos_step(const NvOpDev &op, int prec, const LzState &st, h
 template <int NS>
 static hipError_t normalize_ns(const NvOpDev &op, int prec, const LzState &st, hipStream_t s) {
     const int g = grid_stride_blocks(op.D);
-    if (prec == 0 || st.vlo == nullptr)
-        hipLaunchKernelGGL((normalize_rows_kernel<NS, double>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N,
-                           (int64_t)op.D, st);
-    else if (prec == 1)
-        hipLaunchKernelGGL((normalize_rows_kernel<NS, float>), dim3(g), dim3(kBlock), 0, s, (int64_t)op.N,
-                           (int64_t)op.D, st);
-    else
-        hipLaunchKernelGGL((normalize_rows_kernel<NS, __hip_bfloat16>), dim3(g), dim3(kBlock), 0, s,
-                           (int64_t)op.N, (int64_t)op.D, st);
+    const int64_t N = op.N, D = op.D;
+    if (op.stream_nt) {
+        if (prec == 0 || st.vlo == nullptr)
+            hipLaunchKernelGGL((normalize_rows_kernel<NS, double, true>), dim3(g), dim3(kBlock), 0, s, N, D, st);
+        else if (prec == 1)
+            hipLaunchKernelGGL((normalize_rows_kernel<NS, float, true>), dim3(g), dim3(kBlock), 0, s, N, D, st);
+        else
+            hipLaunchKernelGGL((normalize_rows_kernel<NS, __hip_bfloat16, true>), dim3(g), dim3(kBlock), 0, s, N, D,
+                               st);
+    } else {
+        if (prec == 0 || st.vlo == nullptr)
+            hipLaunchKernelGGL((normalize_rows_kernel<NS, double, false>), dim3(g), dim3(kBlock), 0, s, N, D, st);
+        else if (prec == 1)
+            hipLaunchKernelGGL((normalize_rows_kernel<NS, float, false>), dim3(g), dim3(kBlock), 0, s, N, D, st);
+        else
+            hipLaunchKernelGGL((normalize_rows_kernel<NS, __hip_bfloat16, false>), dim3(g), dim3(kBlock), 0, s, N,
+                               D, st);
+    }
     return hipGetLastError();
 }
 
