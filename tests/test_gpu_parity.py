@@ -341,6 +341,23 @@ def test_batched_graph_iterations_equal_the_reference_loop(prec):
             assert np.array_equal(a, a0) and np.array_equal(b, b0), cuts
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_non_temporal_streaming_keeps_the_bits(prec, monkeypatch):
+    """NV_STREAM_NT=1 (the default from 512 MB of vector) changes only the
+    cache policy of the step's and normalisation's accesses: the same
+    coefficients as plain accesses, to the bit."""
+    p = workload("cfg1")
+    n = 60
+    got = []
+    for nt in ("0", "1"):
+        monkeypatch.setenv("NV_STREAM_NT", nt)
+        with Engine(p) as eng:
+            eng.lanczos_init(niter=n, reorthog=0, prec=prec)
+            eng.run(1, n)
+            got.append(eng.coefficients()[:2])
+    assert np.array_equal(got[0][0], got[1][0]) and np.array_equal(got[0][1], got[1][1])
+
+
 def test_pro_with_noise_keeps_orthogonality():
     p = workload("cfg1")
     with Engine(p) as eng:
